@@ -1,0 +1,274 @@
+/*
+ * tsdf_hip.h — C ABI of the MI355X (gfx950) lidar -> TSDF integration engine.
+ *
+ * Plain C types and pointers only.  Every entry point returns a status code
+ * (TSDF_OK or a negative TSDF_ERR_*); tsdf_last_error() returns a message for
+ * the last failure on the calling thread.  Nothing here calls exit(): the
+ * reference aborts the process on CUDA errors (tsdf_handler.cu:15-22) and
+ * only prints on overflow (tsdf_handler.cu:43-45, :787, :857).
+ *
+ * Reference interfaces cited as  H: tsdf_package/src/cuda/tsdf_handler.cu,
+ * HH: tsdf_package/include/cuda/tsdf_handler.cuh,
+ * CH: tsdf_package/include/cuda/tsdf_container.cuh,
+ * C:  tsdf_package/src/cuda/tsdf_container.cu.
+ *
+ * Threading: an engine owns one HIP stream; calls on one engine must not run
+ * concurrently from several host threads.  Maps of one engine share its block
+ * heap and hash-entry pool.
+ */
+#ifndef TSDF_HIP_H
+#define TSDF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TSDF_OK 0
+#define TSDF_ERR_INVALID_ARG (-1)
+#define TSDF_ERR_HIP (-2)          /* a HIP runtime call failed */
+#define TSDF_ERR_NO_MEMORY (-3)    /* device allocation failed */
+#define TSDF_ERR_CAPACITY (-4)     /* pool / table / batch capacity exceeded */
+#define TSDF_ERR_BUDGET (-5)       /* map admission refused by the heap budget */
+#define TSDF_ERR_DEADLINE (-6)     /* serving: request finished after its deadline */
+
+/* Voxel state formats of an engine's block heap. */
+#define TSDF_STATE_F32 0  /* {float sdf, float weight}: the reference Voxel, CH:40-46 */
+#define TSDF_STATE_F16 1  /* {half sdf/truncation, half weight} packed in 4 bytes   */
+
+typedef struct tsdf_engine tsdf_engine;
+typedef struct tsdf_map tsdf_map;
+
+/* Integration parameters: the five arguments of tsdf::initGlobalVars
+ * (HH:80-81, H:1289-1317).  Derived constants (half sizes, block size,
+ * traversal bounds) are computed from them exactly as initGlobalVars does. */
+typedef struct {
+  float voxel_size;
+  float truncation_distance;
+  float max_weight;
+  float publish_distance_squared;
+  float garbage_collect_distance_squared;
+} tsdf_params;
+
+/* Engine sizing.  Replaces the compile-time NUM_HEAP_BLOCKS, NUM_BUCKETS and
+ * HASH_ENTRIES_PER_BUCKET of CH:23-31 with run-time sizes chosen for HBM. */
+typedef struct {
+  int32_t device;            /* HIP device ordinal */
+  int32_t state_format;      /* TSDF_STATE_F32 or TSDF_STATE_F16 */
+  int64_t heap_blocks;       /* block heap capacity, blocks of 8^3 voxels */
+  int64_t total_buckets;     /* hash buckets (8 entries each) shared by all maps */
+  int64_t max_batch_points;  /* initial scratch capacity in points (grows on demand) */
+} tsdf_engine_config;
+
+/* Per-map hash layout and heap quota (0 in a field selects its default). */
+typedef struct {
+  int32_t buckets_log2;      /* home buckets = 2^buckets_log2 (default 16) */
+  int32_t overflow_buckets;  /* overflow bucket pool (default home/4) */
+  int32_t max_chain;         /* max overflow buckets chained to one home bucket (default 64) */
+  int32_t reserved;
+  int64_t block_quota;       /* max heap blocks this map may hold (default: no quota) */
+} tsdf_map_config;
+
+/* Counters of one integrate / batch call. */
+typedef struct {
+  int64_t points;            /* input points */
+  int64_t valid;             /* points integrated */
+  int64_t skipped;           /* non-finite, zero-range or out-of-range points */
+  int64_t block_cells;       /* block-DDA cells emitted */
+  int64_t voxel_updates;     /* voxel updates folded */
+  int64_t unique_voxels;     /* distinct voxels updated */
+  int64_t new_blocks;        /* blocks allocated */
+  int64_t trunc_vox;         /* points whose voxel DDA hit MAX_VOXELS_TRAVERSED */
+  int64_t trunc_blk;         /* points whose block DDA hit the block bound */
+  int64_t index_dropped;     /* updates whose local voxel index fell outside 0..7 */
+  int64_t insert_failures;   /* block inserts refused: chain or overflow pool full */
+  int64_t heap_failures;     /* blocks refused: heap or quota exhausted */
+  int64_t updates_dropped;   /* updates lost to insert/heap failures */
+  int64_t published;         /* voxels published by the last publish call */
+  int64_t gc_removed;        /* blocks removed by the last gc call */
+  int64_t quota_failures;    /* subset of heap_failures caused by block_quota */
+} tsdf_scan_stats;
+
+/* One scan of a batch. */
+typedef struct {
+  int32_t map_index;         /* index into the maps array of the batch call */
+  int32_t n_points;
+  int64_t point_offset;      /* first point of this scan in the batch point array */
+  float sensor[3];           /* lidar position, the reference's lidar_position_h */
+  int32_t reserved;
+} tsdf_scan_desc;
+
+/* ---- errors --------------------------------------------------------------- */
+const char* tsdf_last_error(void);
+
+/* ---- engine ----------------------------------------------------------------
+ * Replaces: TSDFContainer::TSDFContainer / ~TSDFContainer (C:11-26,
+ * CH:112-133): allocation of the device hash table and block heap.  Unlike
+ * the reference, one engine holds many maps. */
+int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out);
+int tsdf_engine_destroy(tsdf_engine* eng);
+/* Blocks until all work queued on the engine stream has finished. */
+int tsdf_engine_synchronize(tsdf_engine* eng);
+/* The engine's HIP stream (hipStream_t) for callers that order their own work. */
+void* tsdf_engine_stream(tsdf_engine* eng);
+/* Free heap blocks right now (synchronises). */
+int tsdf_engine_free_blocks(tsdf_engine* eng, int64_t* out);
+
+/* ---- maps ------------------------------------------------------------------
+ * Replaces: TSDFHandler::TSDFHandler (H:954-958) together with
+ * initGlobalVars (H:1289-1317): a map owns its parameters instead of
+ * process-global __constant__ symbols (HH:23-37).  cfg may be NULL. */
+int tsdf_map_create(tsdf_engine* eng, const tsdf_params* params, const tsdf_map_config* cfg, tsdf_map** out);
+/* Replaces ~TSDFHandler (H:960-964): returns the map's blocks to the heap. */
+int tsdf_map_destroy(tsdf_map* map);
+int tsdf_map_num_blocks(tsdf_map* map, int64_t* out);
+/* Copies every block of the map to host memory: keys int32[n][3] (integer
+ * block coordinates, floor(x / (8 * voxel_size))) and voxels float[n][512][2]
+ * ({sdf, weight}, local index x + 8y + 64z as in H:507-512).  Writes at most
+ * capacity blocks; *count receives the total.  Either output may be NULL. */
+int tsdf_map_export(tsdf_map* map, int32_t* keys, float* voxels, int64_t capacity, int64_t* count);
+
+/* ---- per-scan path ---------------------------------------------------------
+ * Replaces: TSDFHandler::allocateVoxelBlocksAndUpdateVoxels (H:1027-1060),
+ * i.e. getVoxelBlocks (H:1071-1094), allocateVoxelBlocks (H:1103-1153) and
+ * getVoxelsAndUpdate (H:1164-1182).  points_dev: device array of n
+ * pcl::PointXYZ-layout float4 (x, y, z, pad).  Asynchronous on the engine
+ * stream; the point buffer must stay valid until the work completes. */
+int tsdf_integrate(tsdf_map* map, const float* points_dev, int64_t n, const float sensor[3]);
+
+/* Replaces: TSDFHandler::retrievePublishableVoxels (H:1193-1224).  Writes
+ * every voxel with weight != 0 in blocks whose centre lies within
+ * sqrt(publish_distance_squared) of the sensor: positions float[n][3] and
+ * data float[n][2] = {sdf, weight}, i.e. the reference's Vector3f[] and
+ * Voxel[] output arrays (HH:49-51).  Counts first, so there is no fixed
+ * capacity: *count receives the total, at most capacity entries are written.
+ * Host pointers; synchronous. */
+int tsdf_publish(tsdf_map* map, const float sensor[3], float* pos, float* data, int64_t capacity, int64_t* count);
+
+/* Publishes straight into the tsdf_package_msgs/Voxel wire layout
+ * (tsdf_package_msgs/msg/Voxel.msg: float32 sdf, weight, x, y, z; 20 bytes per
+ * voxel), replacing the host push_back loop of Publisher::publish
+ * (tsdf_package/src/publisher.cpp:13-25).  Host pointer; synchronous. */
+int tsdf_publish_msg(tsdf_map* map, const float sensor[3], float* voxels_msg, int64_t capacity, int64_t* count);
+
+/* Replaces: TSDFHandler::garbageCollectDistantBlocks (H:1232-1279).  Removes
+ * every block farther than sqrt(garbage_collect_distance_squared) from the
+ * sensor and returns its heap block to the free list.  removed may be NULL
+ * (then the call is asynchronous). */
+int tsdf_gc(tsdf_map* map, const float sensor[3], int64_t* removed);
+
+/* Replaces: TSDFHandler::processPointCloudAndUpdateVoxels (H:974-1016): the
+ * whole per-scan sequence integrate -> publish -> gc with host inputs and
+ * outputs.  points_host: n float4 rows; pos/data as for tsdf_publish.
+ * Synchronous. */
+int tsdf_process_scan(tsdf_map* map, const float* points_host, int64_t n, const float sensor[3],
+                      float* pos, float* data, int64_t capacity, int64_t* count);
+
+/* Counters of the last integrate/batch/publish/gc call on this engine (synchronises). */
+int tsdf_last_stats(tsdf_engine* eng, tsdf_scan_stats* out);
+
+/* ---- batched path ------------------------------------------------------------
+ * No reference counterpart (the reference integrates one scan per call).
+ * Integrates nscans scans of possibly different maps in one pass of the
+ * pipeline.  Scans of the same map are folded in array order, so the result
+ * equals calling tsdf_integrate for each scan in order.  points_dev holds all
+ * scans' points (scan i at point_offset).  Asynchronous. */
+int tsdf_integrate_batch(tsdf_engine* eng, tsdf_map* const* maps, int32_t nmaps,
+                         const tsdf_scan_desc* scans, int32_t nscans, const float* points_dev);
+
+/* A batch pipeline captured once into a HIP graph for up to max_points
+ * points and max_scans scans over a fixed set of maps, then replayed with new
+ * scan descriptors and point buffers. */
+typedef struct tsdf_batch_graph tsdf_batch_graph;
+int tsdf_batch_graph_create(tsdf_engine* eng, tsdf_map* const* maps, int32_t nmaps,
+                            int64_t max_points, int32_t max_scans, tsdf_batch_graph** out);
+int tsdf_batch_graph_launch(tsdf_batch_graph* g, const tsdf_scan_desc* scans, int32_t nscans, const float* points_dev);
+int tsdf_batch_graph_destroy(tsdf_batch_graph* g);
+
+/* Launch-shape knobs of the batched pipeline, set by the autotuner. */
+typedef struct {
+  int32_t traverse_block;    /* threads per workgroup of the traversal kernel (64..1024) */
+  int32_t traverse_grid_cap; /* max workgroups of the traversal kernel */
+  int32_t fold_grid;         /* workgroups of the persistent fold kernel */
+  int32_t reserved;
+} tsdf_launch_config;
+int tsdf_engine_set_launch(tsdf_engine* eng, const tsdf_launch_config* cfg);
+int tsdf_engine_get_launch(tsdf_engine* eng, tsdf_launch_config* cfg);
+
+/* ---- timing -----------------------------------------------------------------
+ * When enabled, each pipeline stage is bracketed by HIP events on the engine
+ * stream.  tsdf_timing_read synchronises and returns per-stage totals (ms)
+ * and launch counts since the last reset, for stage ids below. */
+#define TSDF_STAGE_TRAVERSE 0
+#define TSDF_STAGE_ASSIGN 1
+#define TSDF_STAGE_SCAN 2
+#define TSDF_STAGE_SCATTER 3
+#define TSDF_STAGE_FOLD 4
+#define TSDF_STAGE_PUBLISH 5
+#define TSDF_STAGE_GC 6
+#define TSDF_STAGE_TOTAL 7
+#define TSDF_NUM_STAGES 8
+int tsdf_timing_enable(tsdf_engine* eng, int on);
+int tsdf_timing_read(tsdf_engine* eng, double* ms, int64_t* launches, int reset);
+
+/* ---- serving ----------------------------------------------------------------
+ * No reference counterpart.  A continuous-batching request loop: requests
+ * (scans of resident maps) become eligible at their arrival time; while the
+ * device works on earlier batches, newly arrived requests are gathered into
+ * the next batch (at most max_batch_scans scans and max_batch_points points,
+ * one scan per map per batch so that scans of a map keep their order).  Up to
+ * max_inflight batches are queued on the device.  Arrival times are seconds
+ * relative to the start of the run; the loop sleeps until arrivals.
+ * latency_ms[i] receives completion minus arrival of request i, and
+ * batch_of[i] the batch it ran in (either may be NULL). */
+typedef struct {
+  int32_t map_index;         /* index into the maps array */
+  int32_t n_points;
+  int64_t point_offset;      /* into points_dev */
+  float sensor[3];
+  float arrival_s;
+} tsdf_request;
+
+typedef struct {
+  int32_t max_batch_scans;
+  int32_t max_inflight;
+  int64_t max_batch_points;
+  double deadline_ms;        /* requests slower than this are counted as misses (0: none) */
+} tsdf_serve_config;
+
+typedef struct {
+  int64_t requests, batches, deadline_misses;
+  double wall_s;             /* first arrival to last completion */
+  double busy_s;             /* summed device time of the batches */
+  int64_t points;
+} tsdf_serve_stats;
+
+int tsdf_serve_run(tsdf_engine* eng, tsdf_map* const* maps, int32_t nmaps,
+                   const tsdf_request* reqs, int32_t nreq, const float* points_dev,
+                   const tsdf_serve_config* cfg, double* latency_ms, int32_t* batch_of,
+                   tsdf_serve_stats* stats);
+
+/* Map admission against a heap budget: succeeds when the engine's free heap
+ * blocks minus the quotas already promised to admitted maps can cover
+ * cfg->block_quota; otherwise returns TSDF_ERR_BUDGET without creating the map. */
+int tsdf_map_admit(tsdf_engine* eng, const tsdf_params* params, const tsdf_map_config* cfg, tsdf_map** out);
+
+/* ---- device memory plumbing ---------------------------------------------------
+ * No reference counterpart: lets hosts without their own GPU runtime stage
+ * point clouds in HBM.  stream-ordered on the engine stream. */
+int tsdf_device_alloc(tsdf_engine* eng, int64_t bytes, void** out);
+int tsdf_device_free(tsdf_engine* eng, void* ptr);
+int tsdf_copy_to_device(tsdf_engine* eng, void* dst, const void* src, int64_t bytes);
+int tsdf_copy_to_host(tsdf_engine* eng, void* dst, const void* src, int64_t bytes);
+
+/* Query of the derived constants initGlobalVars computes (H:1296-1315):
+ * out_f = {half_voxel, block_size, half_block, block_eps, voxel_eps},
+ * out_i = {MAX_VOXELS_TRAVERSED_PER_LIDAR_POINT, max_voxel_blocks_traversed}. */
+int tsdf_derived_params(const tsdf_params* params, float* out_f, int32_t* out_i);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSDF_HIP_H */

@@ -1,0 +1,9 @@
+"""MI355X-native lidar -> TSDF integration engine.
+
+The compute path is the HIP library built from ros_tsdf_amd/csrc (loaded by
+ros_tsdf_amd.abi); this package holds the Python host interface over its C
+ABI, the reference-shaped handler (handler.TSDFHandler), workload
+generation and the batched / serving / autotuning drivers.
+"""
+from . import abi  # noqa: F401
+from .engine import DeviceBuffer, Engine, Map, derived_params  # noqa: F401

@@ -1,0 +1,163 @@
+// Device data structures of the engine: the spatial hash (buckets of 8
+// entries with overflow chains), the block heap and its free list, per-call
+// counters, and the scratch layout of one integration pass.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "geom.hpp"
+
+namespace tsdfhip {
+
+constexpr uint64_t kEmptyKey = ~0ull;  // packed keys use 63 bits, so never all ones
+constexpr int kBucket = 8;             // entries per bucket (one 64-byte line of keys)
+constexpr int kMaxScansPerBatch = 256;
+constexpr int kPointIdxBits = 23;      // point index inside one pass
+constexpr int kMaxPassPoints = (1 << kPointIdxBits) - 2;
+
+// Statistic slots (EngCtr::stats), mirrored by tsdf_scan_stats.
+enum Stat {
+  ST_POINTS = 0, ST_VALID, ST_SKIPPED, ST_BLOCK_CELLS, ST_UPDATES, ST_UNIQUE, ST_NEW_BLOCKS, ST_TRUNC_VOX,
+  ST_TRUNC_BLK, ST_INDEX_DROPPED, ST_INSERT_FAIL, ST_HEAP_FAIL, ST_UPD_DROPPED, ST_PUBLISHED, ST_GC_REMOVED,
+  ST_QUOTA_FAIL, ST_COUNT
+};
+
+// Per-map constants, resident in device memory (one entry per map slot).
+struct MapDev {
+  float vs, T, maxW, pubD2, gcD2;
+  float half_vs, BL, half_BL;
+  int32_t max_vox, max_blk;
+  uint32_t home_mask;     // home buckets - 1 (power of two)
+  int32_t bucket_base;    // global index of home bucket 0
+  int32_t n_home, n_ovf;  // overflow buckets live at [bucket_base + n_home, + n_ovf)
+  int32_t max_chain;
+  int32_t slot;
+  int64_t block_quota;    // 0 = none
+};
+
+// Per-map mutable counters.
+struct MapCtr {
+  int32_t ovf_top;    // entries on the overflow-bucket free stack
+  int32_t orphan_n;   // overflow buckets popped by a chain extension that lost its link race
+  int32_t blocks;     // heap blocks held
+  int32_t pad;
+};
+
+// Engine-wide counters.
+struct EngCtr {
+  int32_t heap_top;   // entries on the heap free stack
+  int32_t touched_n;  // blocks with updates in this pass
+  int32_t new_n;      // entries inserted in this pass
+  int32_t sel_n;      // selected blocks (publish / export)
+  int32_t freed_n;    // blocks freed by the current gc
+  int32_t big_n;      // blocks whose update list exceeds one LDS tile
+  int32_t pad[2];
+  unsigned long long stats[ST_COUNT];
+};
+
+struct ScanDev {
+  int32_t map;        // map slot
+  int32_t n;
+  int32_t off;        // first point in the pass
+  float s[3];
+};
+
+struct PassHdr {
+  const float4* points;
+  int32_t total;      // points in the pass
+  int32_t nscans;
+  int32_t stride;     // record stride (max MAX_VOXELS_TRAVERSED of the pass's maps)
+  int32_t pad;
+};
+
+// Everything a kernel may touch, passed by value.
+struct Dev {
+  // engine tables
+  const MapDev* maps;
+  MapCtr* mctr;
+  EngCtr* ctr;
+  unsigned long long* keys;  // [entries]
+  int32_t* vals;             // [entries] heap block of an entry, -1 none
+  uint32_t* blk_cnt;         // [entries] updates routed to the entry in this pass
+  uint32_t* touch_idx;       // [entries] index in the touched list
+  int32_t* next;             // [buckets] overflow link, -1 none
+  int32_t* ovf_stack;        // [buckets] per-map overflow free stacks
+  int32_t* orphans;          // [buckets] per-map orphan lists
+  void* heap;                // [heap_blocks][512] voxel state
+  int32_t* heap_free;        // [heap_blocks] free stack
+  int32_t* freed;            // [heap_blocks] blocks freed by gc (to be zeroed)
+  // pass scratch
+  const PassHdr* hdr;
+  const ScanDev* scans;
+  uint32_t* rec_e;           // [stride][pcap] entry of each update record
+  unsigned long long* rec_kd;  // [stride][pcap] (local << 55 | point << 32 | sdf bits)
+  uint8_t* rec_n;            // [pcap]
+  int64_t pcap;
+  uint32_t* touched;         // [entries]
+  uint32_t* seg_off;         // [entries + 1]
+  uint32_t* tile_sums;       // [entries / 1024 + 2]
+  uint32_t* new_list;        // [entries]
+  int32_t* new_map;          // [entries]
+  uint32_t* big_list;        // [entries]
+  unsigned long long* seg;   // [records]
+  int64_t seg_cap;
+  uint32_t* sel;             // [entries]
+  uint32_t* pub_cnt;         // [entries + 1]
+};
+
+// ---- atomics that bypass the per-CU L1 --------------------------------------
+__device__ __forceinline__ unsigned long long ld_acq(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t ld_acq(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t hash_key(unsigned long long k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return (uint32_t)k;
+}
+
+// ---- voxel state codecs -----------------------------------------------------
+struct StateF32 {
+  using T = float2;
+  static constexpr int kBytes = 8;
+  __device__ static __forceinline__ void load(const void* heap, int64_t i, float inv_T, float Tt, float& S, float& W) {
+    const float2 v = reinterpret_cast<const float2*>(heap)[i];
+    S = v.x;
+    W = v.y;
+  }
+  __device__ static __forceinline__ void store(void* heap, int64_t i, float inv_T, float Tt, float S, float W) {
+    reinterpret_cast<float2*>(heap)[i] = make_float2(S, W);
+  }
+  __device__ static __forceinline__ bool nonzero(const void* heap, int64_t i) {
+    return reinterpret_cast<const float2*>(heap)[i].y != 0.0f;
+  }
+  __device__ static __forceinline__ void zero(void* heap, int64_t i) { reinterpret_cast<float2*>(heap)[i] = make_float2(0.f, 0.f); }
+};
+
+// FP16 SDF normalised by the truncation distance plus FP16 weight, 4 bytes.
+struct StateF16 {
+  static constexpr int kBytes = 4;
+  __device__ static __forceinline__ void load(const void* heap, int64_t i, float inv_T, float Tt, float& S, float& W) {
+    const uint32_t v = reinterpret_cast<const uint32_t*>(heap)[i];
+    S = __half2float(__ushort_as_half((unsigned short)(v & 0xFFFFu))) * Tt;
+    W = __half2float(__ushort_as_half((unsigned short)(v >> 16)));
+  }
+  __device__ static __forceinline__ void store(void* heap, int64_t i, float inv_T, float Tt, float S, float W) {
+    const unsigned short s = __half_as_ushort(__float2half_rn(S / Tt));
+    const unsigned short w = __half_as_ushort(__float2half_rn(W));
+    reinterpret_cast<uint32_t*>(heap)[i] = (uint32_t)s | ((uint32_t)w << 16);
+  }
+  __device__ static __forceinline__ bool nonzero(const void* heap, int64_t i) {
+    return (reinterpret_cast<const uint32_t*>(heap)[i] >> 16) != 0u;
+  }
+  __device__ static __forceinline__ void zero(void* heap, int64_t i) { reinterpret_cast<uint32_t*>(heap)[i] = 0u; }
+};
+
+}  // namespace tsdfhip

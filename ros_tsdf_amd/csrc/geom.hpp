@@ -1,0 +1,211 @@
+// Device-side FP32 geometry of the integration path.
+//
+// Every expression keeps the evaluation order of the reference
+// (tsdf_package/src/cuda/tsdf_handler.cu, "H:") so that cell keys, voxel
+// centres and the discrete decisions (DDA steps, sign of the SDF, publish and
+// GC radius tests) are bit-identical to an IEEE FP32 evaluation on any
+// machine.  This translation unit is compiled with -ffp-contract=off, so no
+// product is fused into an add, and FP32 '/' and sqrtf are correctly rounded.
+// pow(x, 2) of the reference is x * x.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tsdfhip {
+
+constexpr int kVoxelsPerSide = 8;    // VOXELS_PER_SIDE, tsdf_container.cuh:19
+constexpr int kVoxelsPerBlock = 512; // VOXELS_PER_BLOCK, tsdf_container.cuh:21
+
+// Integer cell coordinates are packed 21 bits per axis with a 2^20 bias.
+// The point validity rule (|coord| <= 2^16 voxels) keeps every key in range.
+__host__ __device__ inline uint64_t pack_key(int32_t x, int32_t y, int32_t z) {
+  const uint64_t b = 1u << 20;
+  return ((uint64_t)(uint32_t)(x + (int32_t)b) << 42) | ((uint64_t)(uint32_t)(y + (int32_t)b) << 21) |
+         (uint64_t)(uint32_t)(z + (int32_t)b);
+}
+__host__ __device__ inline void unpack_key(uint64_t k, int32_t* o) {
+  const int32_t b = 1 << 20;
+  o[0] = (int32_t)((k >> 42) & 0x1FFFFF) - b;
+  o[1] = (int32_t)((k >> 21) & 0x1FFFFF) - b;
+  o[2] = (int32_t)(k & 0x1FFFFF) - b;
+}
+
+// floorFun + getVolumeCenterFromPoint (H:65-85) for one axis: returns the
+// cell centre floor(x * s) / s + L/2 with s = 1/L, and the integer floor.
+__device__ __forceinline__ float cell_center(float x, float L, int32_t& k) {
+  const float scale = 1.0f / L;
+  const float half = L / 2.0f;
+  const float f = floorf(x * scale);
+  k = (int32_t)f;
+  return f / scale + half;
+}
+
+// Point validity (the engine's rule for inputs the reference cannot handle:
+// H:168 never terminates on NaN, H:119 divides by a zero range).
+__device__ __forceinline__ bool point_valid(const float p[3], const float s[3], float vs) {
+  const float lim = vs * 65536.0f;
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    ok = ok && isfinite(p[i]) && isfinite(s[i]) && fabsf(p[i]) <= lim && fabsf(s[i]) <= lim;
+  }
+  if (!ok) return false;
+  const float v0 = p[0] - s[0], v1 = p[1] - s[1], v2 = p[2] - s[2];
+  const float m = sqrtf(v0 * v0 + v1 * v1 + v2 * v2);
+  return m > 0.0f;
+}
+
+// getTruncationLineEndPoints (H:112-133): the segment p -+ T * (p - s)/|p - s|,
+// ordered so that a is the end nearer the sensor.
+__device__ __forceinline__ void truncation_segment(const float p[3], const float s[3], float T, float a[3], float b[3]) {
+  const float v0 = p[0] - s[0], v1 = p[1] - s[1], v2 = p[2] - s[2];
+  const float m = sqrtf(v0 * v0 + v1 * v1 + v2 * v2);
+  const float n[3] = {v0 / m, v1 / m, v2 / m};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float t = T * n[i];
+    a[i] = p[i] - t;
+    b[i] = p[i] + t;
+  }
+  const float a0 = a[0] - s[0], a1 = a[1] - s[1], a2 = a[2] - s[2];
+  const float b0 = b[0] - s[0], b1 = b[1] - s[1], b2 = b[2] - s[2];
+  const float da = a0 * a0 + a1 * a1 + a2 * a2;
+  const float db = b0 * b0 + b1 * b1 + b2 * b2;
+  if (db < da) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float t = a[i];
+      a[i] = b[i];
+      b[i] = t;
+    }
+  }
+}
+
+// traverseVolume (H:146-245): steps a point from a towards b in increments of
+// +-L, one or more axes at a time by the reference's tMax comparison tree,
+// re-deriving the cell centre after each step; stops when the centre matches
+// b's cell within L/4 (emitting it), or without emitting when a step moved no
+// axis by a full cell or some axis by more than one (the reference's precision
+// guard).  At most max_cells cells are emitted; *trunc reports the cut.
+// emit(k0, k1, k2, c0, c1, c2) receives each cell's integer key and centre.
+template <class Emit>
+__device__ __forceinline__ int traverse(const float a[3], const float b[3], float L, int max_cells, bool& trunc,
+                                        Emit&& emit) {
+  trunc = false;
+  const float v[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const float half = L / 2.0f;
+  const float eps = L / 4.0f;
+  const float Lpe = L + eps;
+  const float Lme = L - eps;
+  float step[3], tMax[3], tDelta[3], cur[3], cc[3], ce[3];
+  int32_t kc[3], ke[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    step[i] = v[i] > 0 ? L : -1.0f * L;
+    int32_t kd;
+    const float c0 = cell_center(a[i], L, kd);
+    tMax[i] = fabsf(v[i] < 0 ? (c0 - half - a[i]) / v[i] : (c0 + half - a[i]) / v[i]);
+    tDelta[i] = fabsf(L / v[i]);
+    cur[i] = a[i];
+    cc[i] = cell_center(cur[i], L, kc[i]);
+    ce[i] = cell_center(b[i], L, ke[i]);
+  }
+  int n = 0;
+  while (!(fabsf(cc[0] - ce[0]) < eps && fabsf(cc[1] - ce[1]) < eps && fabsf(cc[2] - ce[2]) < eps)) {
+    if (n == max_cells) {
+      trunc = true;
+      return n;
+    }
+    emit(kc[0], kc[1], kc[2], cc[0], cc[1], cc[2]);
+    ++n;
+    bool sx = false, sy = false, sz = false;
+    if (tMax[0] < tMax[1]) {
+      if (tMax[0] < tMax[2]) sx = true;
+      else if (tMax[0] > tMax[2]) sz = true;
+      else { sx = true; sz = true; }
+    } else if (tMax[0] > tMax[1]) {
+      if (tMax[1] < tMax[2]) sy = true;
+      else if (tMax[1] > tMax[2]) sz = true;
+      else { sy = true; sz = true; }
+    } else {
+      if (tMax[2] < tMax[0]) sz = true;
+      else if (tMax[2] > tMax[0]) { sx = true; sy = true; }
+      else { sx = true; sy = true; sz = true; }
+    }
+    if (sx) { cur[0] += step[0]; tMax[0] += tDelta[0]; }
+    if (sy) { cur[1] += step[1]; tMax[1] += tDelta[1]; }
+    if (sz) { cur[2] += step[2]; tMax[2] += tDelta[2]; }
+    float pc[3] = {cc[0], cc[1], cc[2]};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) cc[i] = cell_center(cur[i], L, kc[i]);
+    const float d0 = fabsf(pc[0] - cc[0]);
+    const float d1 = fabsf(pc[1] - cc[1]);
+    const float d2 = fabsf(pc[2] - cc[2]);
+    if ((d0 < Lme && d1 < Lme && d2 < Lme) || (d0 > Lpe || d1 > Lpe || d2 > Lpe)) return n;
+  }
+  if (n == max_cells) {
+    trunc = true;
+    return n;
+  }
+  emit(kc[0], kc[1], kc[2], cc[0], cc[1], cc[2]);
+  return n + 1;
+}
+
+// Voxel centre -> block key and local voxel index: getVolumeCenterFromPoint
+// with the block size (H:612), bottom-left corner (H:622-624) and
+// getLocalVoxelIndex (H:507-512).  Returns false when a floor leaves 0..7.
+__device__ __forceinline__ bool voxel_to_block(const float vc[3], float vs, float BL, float half_BL, int32_t kb[3],
+                                               int& local) {
+  float f[3];
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float bc = cell_center(vc[i], BL, kb[i]);
+    const float bl = bc - half_BL;
+    float d = vc[i] - bl;
+    d = d / vs;
+    f[i] = floorf(d);
+    ok = ok && (f[i] >= 0.0f) && (f[i] <= 7.0f);
+  }
+  const float idx = f[0] + (f[1] * (float)kVoxelsPerSide) + (f[2] * (float)kVoxelsPerSide * (float)kVoxelsPerSide);
+  local = (int)idx;
+  return ok;
+}
+
+// getDistanceUpdate (H:562-573): |p - x|, negated when (p - s).(p - x) < 0.
+__device__ __forceinline__ float signed_distance(const float p[3], const float s[3], const float x[3]) {
+  const float ps0 = p[0] - s[0], ps1 = p[1] - s[1], ps2 = p[2] - s[2];
+  const float px0 = p[0] - x[0], px1 = p[1] - x[1], px2 = p[2] - x[2];
+  const float mag = sqrtf(px0 * px0 + px1 * px1 + px2 * px2);
+  const float dot = ps0 * px0 + ps1 * px1 + ps2 * px2;
+  return dot < 0 ? -mag : mag;
+}
+
+// Block centre from its integer key (the value getVolumeCenterFromPoint
+// produced when it derived that key).
+__device__ __forceinline__ void block_center(const int32_t k[3], float BL, float c[3]) {
+  const float scale = 1.0f / BL;
+  const float half = BL / 2.0f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) c[i] = (float)k[i] / scale + half;
+}
+
+// withinDistanceSquared (H:749-757) with a = block centre, b = sensor.
+__device__ __forceinline__ float dist2(const float a[3], const float b[3]) {
+  const float d0 = b[0] - a[0], d1 = b[1] - a[1], d2 = b[2] - a[2];
+  return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+// getVoxelCoordinatesFromLocalVoxelIndex (H:520-534).
+__device__ __forceinline__ void voxel_position(int idx, const float bl[3], float vs, float half_vs, float out[3]) {
+  const float z = (float)(idx / (kVoxelsPerSide * kVoxelsPerSide));
+  idx = (int)((float)idx - z * (float)kVoxelsPerSide * (float)kVoxelsPerSide);
+  const float y = (float)(idx / kVoxelsPerSide);
+  idx = (int)((float)idx - y * (float)kVoxelsPerSide);
+  const float x = (float)idx;
+  out[0] = x * vs + half_vs + bl[0];
+  out[1] = y * vs + half_vs + bl[1];
+  out[2] = z * vs + half_vs + bl[2];
+}
+
+}  // namespace tsdfhip

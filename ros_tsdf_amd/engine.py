@@ -1,0 +1,214 @@
+"""Python host interface over the C ABI: engines, maps, device buffers."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import check
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(abi.fp)
+
+
+def params_struct(voxel_size, truncation_distance, max_weight, publish_distance_squared,
+                  garbage_collect_distance_squared) -> abi.Params:
+    return abi.Params(voxel_size, truncation_distance, max_weight, publish_distance_squared,
+                      garbage_collect_distance_squared)
+
+
+class DeviceBuffer:
+    """A block of HBM owned by an engine (stream-ordered copies)."""
+
+    def __init__(self, engine: "Engine", nbytes: int):
+        self.engine = engine
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(abi.load().tsdf_device_alloc(engine.h, max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, engine: "Engine", a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(engine, a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray, offset: int = 0):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        check(abi.load().tsdf_copy_to_device(self.engine.h, C.c_void_p(self.ptr + offset), a.ctypes.data, a.nbytes))
+
+    def download(self, out: np.ndarray, offset: int = 0):
+        assert offset + out.nbytes <= self.nbytes
+        check(abi.load().tsdf_copy_to_host(self.engine.h, out.ctypes.data, C.c_void_p(self.ptr + offset), out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.engine.h:
+            check(abi.load().tsdf_device_free(self.engine.h, C.c_void_p(self.ptr)))
+        self.ptr = None
+
+
+class Engine:
+    """Device block heap + hash-entry pool shared by its maps; one HIP stream."""
+
+    def __init__(self, heap_blocks=1 << 18, total_buckets=1 << 20, state_format=abi.TSDF_STATE_F32,
+                 max_batch_points=1 << 17, device=0):
+        cfg = abi.EngineConfig(device, state_format, heap_blocks, total_buckets, max_batch_points)
+        h = C.c_void_p()
+        check(abi.load().tsdf_engine_create(C.byref(cfg), C.byref(h)))
+        self.h = h.value
+        self.state_format = state_format
+        self.heap_blocks = heap_blocks
+
+    def close(self):
+        if self.h:
+            abi.load().tsdf_engine_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def synchronize(self):
+        check(abi.load().tsdf_engine_synchronize(self.h))
+
+    def free_blocks(self) -> int:
+        v = C.c_int64()
+        check(abi.load().tsdf_engine_free_blocks(self.h, C.byref(v)))
+        return v.value
+
+    def map(self, params, buckets_log2=16, overflow_buckets=0, max_chain=64, block_quota=0, admit=False) -> "Map":
+        return Map(self, params, buckets_log2, overflow_buckets, max_chain, block_quota, admit)
+
+    def stats(self) -> dict:
+        s = abi.ScanStats()
+        check(abi.load().tsdf_last_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def integrate_batch(self, maps, scans, points: DeviceBuffer):
+        """scans: list of (map_index, n_points, point_offset, sensor)."""
+        arr = (abi.ScanDesc * len(scans))()
+        for i, (mi, n, off, s) in enumerate(scans):
+            arr[i].map_index = mi
+            arr[i].n_points = n
+            arr[i].point_offset = off
+            arr[i].sensor[:] = [float(x) for x in s]
+        mp = (C.c_void_p * len(maps))(*[m.h for m in maps])
+        check(abi.load().tsdf_integrate_batch(self.h, mp, len(maps), arr, len(scans), C.c_void_p(points.ptr)))
+
+    def set_launch(self, traverse_block=256, traverse_grid_cap=65536, fold_grid=2048):
+        cfg = abi.LaunchConfig(traverse_block, traverse_grid_cap, fold_grid, 0)
+        check(abi.load().tsdf_engine_set_launch(self.h, C.byref(cfg)))
+
+    def get_launch(self) -> dict:
+        cfg = abi.LaunchConfig()
+        check(abi.load().tsdf_engine_get_launch(self.h, C.byref(cfg)))
+        return dict(traverse_block=cfg.traverse_block, traverse_grid_cap=cfg.traverse_grid_cap, fold_grid=cfg.fold_grid)
+
+    def timing(self, on: bool):
+        check(abi.load().tsdf_timing_enable(self.h, 1 if on else 0))
+
+    def timing_read(self, reset=True) -> dict:
+        ms = (C.c_double * 8)()
+        n = (C.c_int64 * 8)()
+        check(abi.load().tsdf_timing_read(self.h, ms, n, 1 if reset else 0))
+        return {abi.STAGES[k]: (ms[k], n[k]) for k in range(8)}
+
+    def buffer(self, nbytes) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+
+class Map:
+    def __init__(self, engine: Engine, params, buckets_log2=16, overflow_buckets=0, max_chain=64, block_quota=0,
+                 admit=False):
+        self.engine = engine
+        self.params = tuple(float(x) for x in params)
+        p = params_struct(*self.params)
+        cfg = abi.MapConfig(buckets_log2, overflow_buckets, max_chain, 0, block_quota)
+        h = C.c_void_p()
+        f = abi.load().tsdf_map_admit if admit else abi.load().tsdf_map_create
+        check(f(engine.h, C.byref(p), C.byref(cfg), C.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if self.h and self.engine.h:
+            check(abi.load().tsdf_map_destroy(self.h))
+        self.h = None
+
+    def integrate(self, points: DeviceBuffer, n: int, sensor, offset_points: int = 0):
+        s = np.asarray(sensor, np.float32)
+        check(abi.load().tsdf_integrate(self.h, C.c_void_p(points.ptr + 16 * offset_points), n, _fptr(s)))
+
+    def publish(self, sensor, capacity=None):
+        s = np.asarray(sensor, np.float32)
+        n = C.c_int64()
+        if capacity is None:
+            check(abi.load().tsdf_publish(self.h, _fptr(s), None, None, 0, C.byref(n)))
+            capacity = n.value
+        pos = np.zeros((max(capacity, 1), 3), np.float32)
+        data = np.zeros((max(capacity, 1), 2), np.float32)
+        check(abi.load().tsdf_publish(self.h, _fptr(s), _fptr(pos), _fptr(data), capacity, C.byref(n)))
+        k = min(n.value, capacity)
+        return pos[:k], data[:k], n.value
+
+    def publish_msg(self, sensor, capacity=None):
+        s = np.asarray(sensor, np.float32)
+        n = C.c_int64()
+        if capacity is None:
+            check(abi.load().tsdf_publish_msg(self.h, _fptr(s), None, 0, C.byref(n)))
+            capacity = n.value
+        msg = np.zeros((max(capacity, 1), 5), np.float32)
+        check(abi.load().tsdf_publish_msg(self.h, _fptr(s), _fptr(msg), capacity, C.byref(n)))
+        return msg[:min(n.value, capacity)], n.value
+
+    def gc(self, sensor) -> int:
+        s = np.asarray(sensor, np.float32)
+        r = C.c_int64()
+        check(abi.load().tsdf_gc(self.h, _fptr(s), C.byref(r)))
+        return r.value
+
+    def num_blocks(self) -> int:
+        v = C.c_int64()
+        check(abi.load().tsdf_map_num_blocks(self.h, C.byref(v)))
+        return v.value
+
+    def export(self):
+        """(keys int32[n,3], sdf float32[n,512], weight float32[n,512]) sorted by key."""
+        n = C.c_int64()
+        check(abi.load().tsdf_map_export(self.h, None, None, 0, C.byref(n)))
+        cap = n.value
+        keys = np.zeros((max(cap, 1), 3), np.int32)
+        vox = np.zeros((max(cap, 1), 512, 2), np.float32)
+        check(abi.load().tsdf_map_export(self.h, keys.ctypes.data_as(abi.i32p), _fptr(vox), cap, C.byref(n)))
+        keys, vox = keys[:cap], vox[:cap]
+        order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
+        keys, vox = keys[order], vox[order]
+        return keys, vox[:, :, 0].copy(), vox[:, :, 1].copy()
+
+    def process_scan(self, points: np.ndarray, sensor, capacity=None):
+        """The reference's processPointCloudAndUpdateVoxels sequence with host arrays."""
+        pts = np.ascontiguousarray(points, np.float32)
+        s = np.asarray(sensor, np.float32)
+        n = C.c_int64()
+        if capacity is None:
+            capacity = 1 << 20
+        pos = np.zeros((capacity, 3), np.float32)
+        data = np.zeros((capacity, 2), np.float32)
+        check(abi.load().tsdf_process_scan(self.h, _fptr(pts), pts.shape[0], _fptr(s), _fptr(pos), _fptr(data),
+                                           capacity, C.byref(n)))
+        k = min(n.value, capacity)
+        return pos[:k], data[:k], n.value
+
+
+def derived_params(params):
+    p = params_struct(*params)
+    of = (C.c_float * 5)()
+    oi = (C.c_int32 * 2)()
+    check(abi.load().tsdf_derived_params(C.byref(p), of, oi))
+    return dict(half_vs=of[0], BL=of[1], half_BL=of[2], blk_eps=of[3], vox_eps=of[4], max_vox=oi[0], max_blk=oi[1])

@@ -1,0 +1,56 @@
+// Drives the drop-in tsdf::TSDFHandler exactly as the reference's ROS node
+// does (initGlobalVars -> new TSDFHandler -> per scan
+// processPointCloudAndUpdateVoxels with caller-owned output arrays of
+// PUBLISH_VOXELS_MAX_SIZE entries and a zeroed count).
+//
+//   handler_dropin_test <in.bin> <out.bin>
+// in.bin : int32 nscans, float params[5]; per scan: int32 n, float sensor[3], float points[n][4]
+// out.bin: per scan: int32 count, then count x {float x, y, z, sdf, weight}
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <vector>
+
+#include "tsdf/tsdf_handler.hpp"
+
+int main(int argc, char** argv) {
+  if (argc != 3) {
+    std::fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
+    return 2;
+  }
+  FILE* in = std::fopen(argv[1], "rb");
+  FILE* out = std::fopen(argv[2], "wb");
+  if (!in || !out) return 2;
+  int nscans = 0;
+  float params[5];
+  if (std::fread(&nscans, 4, 1, in) != 1 || std::fread(params, 4, 5, in) != 5) return 2;
+  tsdf::initGlobalVars(params[0], params[1], params[2], params[3], params[4]);
+  tsdf::TSDFHandler* handler = new tsdf::TSDFHandler();
+  std::vector<Vector3f> pos(PUBLISH_VOXELS_MAX_SIZE);
+  std::vector<tsdf::Voxel> data(PUBLISH_VOXELS_MAX_SIZE);
+  for (int k = 0; k < nscans; ++k) {
+    int n = 0;
+    float s[3];
+    if (std::fread(&n, 4, 1, in) != 1 || std::fread(s, 4, 3, in) != 3) return 2;
+    pcl::PointCloud<pcl::PointXYZ>::Ptr cloud(new pcl::PointCloud<pcl::PointXYZ>());
+    cloud->points.resize(n);
+    for (int i = 0; i < n; ++i) {
+      float p[4];
+      if (std::fread(p, 4, 4, in) != 4) return 2;
+      cloud->points[i] = pcl::PointXYZ(p[0], p[1], p[2]);
+    }
+    Vector3f lidar(s[0], s[1], s[2]);
+    int size = 0;
+    handler->processPointCloudAndUpdateVoxels(cloud, &lidar, pos.data(), &size, data.data());
+    std::fwrite(&size, 4, 1, out);
+    for (int i = 0; i < size; ++i) {
+      const float rec[5] = {pos[i](0), pos[i](1), pos[i](2), data[i].sdf, data[i].weight};
+      std::fwrite(rec, 4, 5, out);
+    }
+  }
+  delete handler;
+  std::fclose(in);
+  std::fclose(out);
+  std::printf("ok %d scans\n", nscans);
+  return 0;
+}

@@ -1,0 +1,94 @@
+"""GPU parity: the HIP engine (through the C ABI) against the FP64 oracle.
+
+Bar: block keys, touched-voxel sets and published positions bit-exact; SDF
+within SDF_TOL_T * truncation and weight within WEIGHT_TOL_REL * max_weight
+(ros_tsdf_amd.configs).
+"""
+import numpy as np
+import pytest
+
+from ros_tsdf_amd import configs, synth
+from tests.helpers import compare_maps, compare_publish
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from ros_tsdf_amd import Engine
+    e = Engine(heap_blocks=1 << 17, total_buckets=1 << 19, max_batch_points=1 << 17)
+    yield e
+    e.close()
+
+
+def _tol(params):
+    return configs.SDF_TOL_T, configs.WEIGHT_TOL_REL * params[2]
+
+
+def run_sequence(eng, oracle, params, w, scans, map_kwargs=None, publish=True, gc=True, check_every=1):
+    """Per scan: integrate -> publish -> gc on both sides, comparing as we go."""
+    m = eng.map(params, **(map_kwargs or {}))
+    om = oracle.OracleMap(params)
+    T = params[1]
+    st, wt = _tol(params)
+    buf = None
+    results = []
+    try:
+        for j, (k, pts, s) in enumerate(scans):
+            if buf is None or buf.nbytes < pts.nbytes:
+                if buf is not None:
+                    buf.free()
+                buf = eng.buffer(max(pts.nbytes, 16))
+            buf.upload(pts)
+            m.integrate(buf, pts.shape[0], s)
+            ost = om.integrate(pts, s)
+            gst = eng.stats()
+            assert gst["voxel_updates"] == ost["n_voxel_updates"], (gst, ost)
+            assert gst["unique_voxels"] == ost["n_unique_voxels"], (gst, ost)
+            assert gst["valid"] == ost["n_valid"] and gst["skipped"] == ost["n_skipped"]
+            assert gst["block_cells"] == ost["n_block_cells"]
+            assert gst["insert_failures"] == 0 and gst["heap_failures"] == 0 and gst["updates_dropped"] == 0
+            if publish:
+                gpos, gdata, n = m.publish(s)
+                opos, osdf, ow = om.publish(s)
+                compare_publish(gpos, gdata, opos, osdf, ow, T, st, wt)
+            if gc:
+                assert m.gc(s) == om.gc(s)
+            if (j + 1) % check_every == 0:
+                results.append(compare_maps(m, om, T, st, wt))
+        results.append(compare_maps(m, om, T, st, wt))
+    finally:
+        if buf is not None:
+            buf.free()
+        m.close()
+        om.close()
+    return results
+
+
+def test_small_workload_parity(eng, oracle_lib):
+    w = configs.SMALL
+    r = run_sequence(eng, oracle_lib, configs.SMALL_PARAMS, w, synth.scans(w, stop=40), check_every=10)
+    print("small:", r[-1])
+
+
+def test_dense_workload_parity(eng, oracle_lib):
+    w = configs.DENSE
+    r = run_sequence(eng, oracle_lib, configs.DENSE_PARAMS, w, synth.scans(w, stop=3))
+    print("dense:", r[-1])
+
+
+def test_tiny_hash_overflow_chains(eng, oracle_lib):
+    """4 home buckets force long overflow chains (and chain-extension races)."""
+    w = configs.SMALL
+    r = run_sequence(eng, oracle_lib, configs.SMALL_PARAMS, w, synth.scans(w, stop=8),
+                     map_kwargs=dict(buckets_log2=2, overflow_buckets=4096, max_chain=4096), check_every=4)
+    print("tiny-hash:", r[-1])
+
+
+def test_gc_recycles_blocks(eng, oracle_lib):
+    """A 30 m GC radius removes blocks every scan; freed heap blocks are reused zeroed."""
+    params = (0.5, 0.1, 1000.0, 2500.0, 900.0)
+    w = configs.SMALL
+    free0 = eng.free_blocks()
+    run_sequence(eng, oracle_lib, params, w, synth.scans(w, stop=30), check_every=5)
+    assert eng.free_blocks() == free0
