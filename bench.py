@@ -1,0 +1,359 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X lidar -> TSDF engine.
+
+Headline (``value``): fused points/s of the plain per-scan path on the dense
+workload (200 scans x 131,072 points, voxel 0.1 m, truncation 0.4 m, one scan
+per launch, synchronised after every scan like the reference API), with the
+scans already resident in HBM.  A "step" is the integration of one scan.
+
+Also reported on the same JSON line:
+  latency_ms   p50/p99 of the full per-scan call (H2D, integrate, publish with
+               D2H of every published voxel, GC), plain path
+  batched      32 scans from 8 maps per HIP-graph launch vs the same scans
+               through the plain per-scan path
+  fp16         the dense workload with 4-byte voxel state, and its error
+               against the FP64 oracle
+  serving      Poisson arrivals over 16 maps at 400 scans/s: continuous
+               batching vs plain per-scan serving, p50/p99 latency
+  roofline     the dominant kernel against MI355X HBM bandwidth
+  cpu_baseline the oracle's multithreaded CPU path on a bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "points/sec fused + p50/p99 scan latency (served vs plain per-scan), 1 MI355X"
+PEAK_HBM = 8.0e12  # bytes/s, MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pct(x, q):
+    return float(np.percentile(np.asarray(x), q)) if len(x) else None
+
+
+def make_pool(scans):
+    from ros_tsdf_amd.workloads import Pool
+    return Pool(scans)
+
+
+def plain_throughput(eng, m, buf, pool, idx, with_stats=True):
+    """Integrates pool scans idx one at a time, synchronising after each.
+    Returns (seconds summed over steps, points, per-step stats)."""
+    total = 0.0
+    pts = 0
+    stats = []
+    for i in idx:
+        eng.synchronize()
+        t0 = time.perf_counter()
+        m.integrate(buf, pool.counts[i], pool.sensors[i], offset_points=pool.offsets[i])
+        eng.synchronize()
+        total += time.perf_counter() - t0
+        pts += pool.counts[i]
+        if with_stats:
+            stats.append(eng.stats())
+    return total, pts, stats
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200, help="timed scans of the dense workload")
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--only", default="all", choices=["all", "plain"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--error-scans", type=int, default=40, help="dense scans for the FP16/FP32 error figures")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the engine shards by map and every workload here fits one MI355X; "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    from ros_tsdf_amd import configs, workloads
+
+    K, W = args.steps, args.warmup
+    n_dense = configs.DENSE.n_scans
+    t_gen = time.perf_counter()
+    need = min(n_dense, K + W)
+    dpts, dsens, fills = workloads.generate(configs.DENSE, range(need))
+    dense = [(dpts[k], dsens[k], 0) for k in range(need)]
+    extras = args.only == "all"
+    if extras:
+        bmaps_pts, bmaps_sens = workloads.generate_maps(configs.DENSE, configs.BATCH_MAPS, 16)
+        smaps_pts, smaps_sens = workloads.generate_maps(configs.DENSE, configs.SERVE_MAPS, 24)
+    log("generated workloads in %.1fs (dense miss-fill fraction %.2f)" % (time.perf_counter() - t_gen, float(np.mean(fills))))
+
+    from ros_tsdf_amd import BatchGraph, Engine, abi, serve
+
+    result = {}
+    # ---------------- plain per-scan, FP32 state ----------------
+    dpool = make_pool(dense)
+    eng = Engine(heap_blocks=1 << 20, total_buckets=1 << 21, max_batch_points=1 << 17)
+    m = eng.map(configs.DENSE_PARAMS, buckets_log2=17)
+    buf = eng.buffer(dpool.points.nbytes)
+    buf.upload(dpool.points)
+    order = [k % need for k in range(W + K)]
+    plain_throughput(eng, m, buf, dpool, order[:W], with_stats=False)
+    eng.timing(True)
+    eng.timing_read(reset=True)
+    secs, npts, stats = plain_throughput(eng, m, buf, dpool, order[W:], with_stats=True)
+    tim = eng.timing_read(reset=True)
+    eng.timing(False)
+    value = npts / secs
+    ms_per_step = 1e3 * secs / K
+    U = np.array([s["unique_voxels"] for s in stats], np.float64)
+    N = np.array([s["valid"] for s in stats], np.float64)
+    M = np.array([s["voxel_updates"] for s in stats], np.float64)
+    bpt = 16.0 + 2.0 * 8.0 * U / N
+    alg_bytes = float(np.mean(16.0 * N + 16.0 * U))
+    stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in tim.items()}
+    kernel_stages = ["traverse", "assign", "scan", "scatter", "fold"]
+    dom = max(kernel_stages, key=lambda k: stage_ms[k])
+    dom_kernel = {"traverse": "k_traverse", "assign": "k_assign", "scan": "k_scan_*", "scatter": "k_scatter",
+                  "fold": "k_fold"}[dom]
+    achieved = alg_bytes / (stage_ms[dom] * 1e-3)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_dense.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as f:
+                traffic = json.load(f).get(dom_kernel)
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
+                "alg_bytes_per_launch": alg_bytes, "bytes_per_point": float(np.mean(bpt)),
+                "pass_achieved_GBps": alg_bytes / (stage_ms["total"] * 1e-3) / 1e9}
+    log("plain: %.3e pts/s, %.3f ms/scan, stages %s" % (value, ms_per_step, {k: round(v, 4) for k, v in stage_ms.items()}))
+    result["stages_ms"] = {k: round(v, 5) for k, v in stage_ms.items()}
+    result["per_scan"] = {"points": float(np.mean(N)), "voxel_updates": float(np.mean(M)),
+                          "unique_voxels": float(np.mean(U)), "blocks_after": m.num_blocks()}
+    plain_lat = None
+    if extras:
+        # ------------- full per-scan call latency (reference sequence) -------------
+        m2 = eng.map(configs.DENSE_PARAMS, buckets_log2=17)
+        lat = []
+        npub = []
+        for k in range(min(30, need)):
+            t0 = time.perf_counter()
+            pos, data, n = m2.process_scan(dense[k][0], dense[k][1], capacity=1 << 23)
+            lat.append(1e3 * (time.perf_counter() - t0))
+            npub.append(n)
+        plain_lat = {"p50": pct(lat[2:], 50), "p99": pct(lat[2:], 99), "published_voxels_mean": float(np.mean(npub)),
+                     "scans": len(lat) - 2}
+        m2.close()
+        log("full-call latency", plain_lat)
+    m.close()
+    buf.free()
+    eng.close()
+
+    if extras:
+        # ---------------- batched: HIP graph, 32 scans from 8 maps ----------------
+        plan = workloads.batch_plan(n_batches=8, scans_per_map=16)
+        batches = []
+        scans_all = []
+        for b in plan:
+            sc = []
+            for (mi, k, n) in b:
+                sc.append((bmaps_pts[mi][k][:n], bmaps_sens[mi][k], mi))
+            batches.append(sc)
+            scans_all.extend(sc)
+        bpool = make_pool(scans_all)
+        eng = Engine(heap_blocks=1 << 21, total_buckets=1 << 21, max_batch_points=1 << 22)
+        maps = [eng.map(configs.DENSE_PARAMS, buckets_log2=16) for _ in range(configs.BATCH_MAPS)]
+        bbuf = eng.buffer(bpool.points.nbytes)
+        bbuf.upload(bpool.points)
+        max_pts = max(sum(s[0].shape[0] for s in b) for b in batches)
+        g = BatchGraph(eng, maps, max_points=max_pts, max_scans=configs.BATCH_SCANS)
+        descs = []
+        j = 0
+        for b in batches:
+            d = []
+            for (p, s, mi) in b:
+                d.append((mi, bpool.counts[j], bpool.offsets[j], bpool.sensors[j]))
+                j += 1
+            descs.append(d)
+        # warm-up on the first batch, then all batches timed
+        g.launch(descs[0], bbuf)
+        eng.synchronize()
+        for mm in maps:
+            mm.close()
+        maps = [eng.map(configs.DENSE_PARAMS, buckets_log2=16) for _ in range(configs.BATCH_MAPS)]
+        g.close()
+        g = BatchGraph(eng, maps, max_points=max_pts, max_scans=configs.BATCH_SCANS)
+        g.launch(descs[0], bbuf)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        for d in descs[1:]:
+            g.launch(d, bbuf)
+            eng.synchronize()
+        tb = time.perf_counter() - t0
+        bpts = sum(sum(x[1] for x in d) for d in descs[1:])
+        batched_pps = bpts / tb
+        # the same scans through the plain per-scan path on fresh maps
+        for mm in maps:
+            mm.close()
+        g.close()
+        maps = [eng.map(configs.DENSE_PARAMS, buckets_log2=16) for _ in range(configs.BATCH_MAPS)]
+        j0 = len(descs[0])
+        tp = 0.0
+        for i, d in enumerate(descs):
+            for (mi, n, off, s) in d:
+                eng.synchronize()
+                t0 = time.perf_counter()
+                maps[mi].integrate(bbuf, n, s, offset_points=off)
+                eng.synchronize()
+                if i > 0:
+                    tp += time.perf_counter() - t0
+        plain_b_pps = bpts / tp
+        result["batched"] = {"points_per_s": batched_pps, "plain_per_scan_points_per_s": plain_b_pps,
+                             "speedup_vs_plain": batched_pps / plain_b_pps, "launches": len(descs) - 1,
+                             "scans_per_launch": configs.BATCH_SCANS, "maps": configs.BATCH_MAPS,
+                             "mean_points_per_scan": bpts / ((len(descs) - 1) * configs.BATCH_SCANS),
+                             "ms_per_launch": 1e3 * tb / (len(descs) - 1)}
+        log("batched", result["batched"])
+        for mm in maps:
+            mm.close()
+        bbuf.free()
+        eng.close()
+
+        # ---------------- FP16 voxel state ----------------
+        eng = Engine(heap_blocks=1 << 20, total_buckets=1 << 21, max_batch_points=1 << 17,
+                     state_format=abi.TSDF_STATE_F16)
+        mh = eng.map(configs.DENSE_PARAMS, buckets_log2=17)
+        hbuf = eng.buffer(dpool.points.nbytes)
+        hbuf.upload(dpool.points)
+        plain_throughput(eng, mh, hbuf, dpool, order[:W], with_stats=False)
+        hs, hn, _ = plain_throughput(eng, mh, hbuf, dpool, order[W:], with_stats=False)
+        result["fp16"] = {"points_per_s": hn / hs, "bytes_per_voxel": 4, "bytes_per_voxel_update_rw": 8}
+        mh.close()
+        # error after error-scans scans, both formats, against the oracle
+        E = min(args.error_scans, need)
+        import oracle
+        om = oracle.OracleMap(configs.DENSE_PARAMS)
+        mh = eng.map(configs.DENSE_PARAMS, buckets_log2=17)
+        eng32 = Engine(heap_blocks=1 << 20, total_buckets=1 << 21, max_batch_points=1 << 17)
+        m32 = eng32.map(configs.DENSE_PARAMS, buckets_log2=17)
+        buf32 = eng32.buffer(dpool.points.nbytes)
+        buf32.upload(dpool.points)
+        nthreads = min(16, os.cpu_count() or 1)
+        tcpu = 0.0
+        cpu_pts = 0
+        cpu_scans = 0
+        for k in range(E):
+            mh.integrate(hbuf, dpool.counts[k], dpool.sensors[k], offset_points=dpool.offsets[k])
+            m32.integrate(buf32, dpool.counts[k], dpool.sensors[k], offset_points=dpool.offsets[k])
+            t0 = time.perf_counter()
+            om.integrate(dense[k][0], dense[k][1], threads=nthreads)
+            dt = time.perf_counter() - t0
+            if tcpu < args.cpu_seconds:
+                tcpu += dt
+                cpu_pts += dpool.counts[k]
+                cpu_scans += 1
+        ok, os_, ow = om.export()
+        T = configs.DENSE_PARAMS[1]
+        errs = {}
+        for name, mm in (("fp16", mh), ("fp32", m32)):
+            gk, gs, gw = mm.export()
+            same = gk.shape == ok.shape and np.array_equal(gk, ok)
+            t = ow != 0
+            e = np.abs(gs[t].astype(np.float64) - os_[t]) / T if same else np.array([np.nan])
+            we = np.abs(gw[t].astype(np.float64) - ow[t]) if same else np.array([np.nan])
+            errs[name] = {"keys_bit_exact": bool(same), "touched_bit_exact": bool(same and np.array_equal(gw != 0, t)),
+                          "max_sdf_err_T": float(e.max()), "rms_sdf_err_T": float(np.sqrt(np.mean(e * e))),
+                          "max_weight_err": float(we.max()), "scans": E}
+        result["fp16"]["error_vs_oracle"] = errs["fp16"]
+        result["fp32_error_vs_oracle"] = errs["fp32"]
+        log("errors", errs)
+        cpu_baseline = {"value": cpu_pts / tcpu, "unit": "points/s", "cores": nthreads, "kind": "port",
+                        "sample": "oracle multithreaded path, first %d dense scans (%d points), %.1f s" % (
+                            cpu_scans, cpu_pts, tcpu)}
+        om.close()
+        mh.close()
+        m32.close()
+        hbuf.free()
+        buf32.free()
+        eng32.close()
+        eng.close()
+
+        # ---------------- serving ----------------
+        budget_blocks = configs.SERVE_HEAP_BUDGET // (512 * 8)
+        eng = Engine(heap_blocks=budget_blocks, total_buckets=1 << 21, max_batch_points=1 << 22)
+        sc_all = []
+        for mi in range(configs.SERVE_MAPS):
+            for k in range(24):
+                sc_all.append((smaps_pts[mi][k], smaps_sens[mi][k], mi))
+        spool = make_pool(sc_all)
+        sbuf = eng.buffer(spool.points.nbytes)
+        sbuf.upload(spool.points)
+        serving = {}
+        for label, counts in (("loguniform", "loguniform"), ("fixed131072", 131072)):
+            reqs_raw = workloads.poisson_requests(args.serve_requests, configs.SERVE_RATE, configs.SERVE_MAPS,
+                                                  counts=counts, scans_per_map=24)
+            reqs = []
+            for (t, mi, k, n) in reqs_raw:
+                j = mi * 24 + k
+                reqs.append((t, mi, min(n, spool.counts[j]), spool.offsets[j], spool.sensors[j]))
+            out = {}
+            for mode, (mbs, mif) in (("served", (32, 2)), ("plain", (1, 1))):
+                quota = budget_blocks // configs.SERVE_MAPS
+                maps = [eng.map(configs.DENSE_PARAMS, buckets_log2=16, block_quota=quota, admit=True)
+                        for _ in range(configs.SERVE_MAPS)]
+                lat, bof, st = serve(eng, maps, reqs, sbuf, max_batch_scans=mbs, max_inflight=mif)
+                pts = sum(r[2] for r in reqs)
+                out[mode] = {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "mean_ms": float(np.mean(lat)),
+                             "points_per_s": pts / st["wall_s"] if st["wall_s"] > 0 else None,
+                             "batches": int(st["batches"]), "requests": int(st["requests"])}
+                for mm in maps:
+                    mm.close()
+            out["rate_scans_per_s"] = configs.SERVE_RATE
+            out["maps"] = configs.SERVE_MAPS
+            serving[label] = out
+            log("serving", label, out)
+        result["serving"] = serving
+        sbuf.free()
+        eng.close()
+    else:
+        cpu_baseline = None
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "dense plain per-scan: 200 scans x 131072 pts (64x2048 lidar), voxel 0.1 m, "
+                               "truncation 0.4 m, GC 150 m, 8 B/voxel, one scan per launch, inputs in HBM",
+                   "scans_timed": K},
+        "roofline": roofline,
+        "cpu_baseline": cpu_baseline,
+        "latency_ms": {"plain_full_call": plain_lat,
+                       "served": result.get("serving", {}).get("loguniform", {}).get("served"),
+                       "plain_served": result.get("serving", {}).get("loguniform", {}).get("plain")},
+    }
+    line.update(result)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

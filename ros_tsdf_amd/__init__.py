@@ -6,4 +6,4 @@ ABI, the reference-shaped handler (handler.TSDFHandler), workload
 generation and the batched / serving / autotuning drivers.
 """
 from . import abi  # noqa: F401
-from .engine import DeviceBuffer, Engine, Map, derived_params  # noqa: F401
+from .engine import BatchGraph, DeviceBuffer, Engine, Map, derived_params, serve  # noqa: F401

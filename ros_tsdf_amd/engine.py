@@ -212,3 +212,61 @@ def derived_params(params):
     oi = (C.c_int32 * 2)()
     check(abi.load().tsdf_derived_params(C.byref(p), of, oi))
     return dict(half_vs=of[0], BL=of[1], half_BL=of[2], blk_eps=of[3], vox_eps=of[4], max_vox=oi[0], max_blk=oi[1])
+
+
+def _scan_array(scans):
+    arr = (abi.ScanDesc * max(len(scans), 1))()
+    for i, (mi, n, off, s) in enumerate(scans):
+        arr[i].map_index = int(mi)
+        arr[i].n_points = int(n)
+        arr[i].point_offset = int(off)
+        arr[i].sensor[:] = [float(x) for x in s]
+    return arr
+
+
+class BatchGraph:
+    """The batched pipeline captured in a HIP graph for a fixed set of maps."""
+
+    def __init__(self, engine: Engine, maps, max_points: int, max_scans: int):
+        self.engine = engine
+        self.maps = list(maps)
+        mp = (C.c_void_p * len(self.maps))(*[m.h for m in self.maps])
+        h = C.c_void_p()
+        check(abi.load().tsdf_batch_graph_create(engine.h, mp, len(self.maps), int(max_points), int(max_scans),
+                                                 C.byref(h)))
+        self.h = h.value
+
+    def launch(self, scans, points: DeviceBuffer):
+        """scans: (map_index, n_points, point_offset, sensor); contiguous and in order."""
+        arr = _scan_array(scans)
+        check(abi.load().tsdf_batch_graph_launch(self.h, arr, len(scans), C.c_void_p(points.ptr)))
+
+    def close(self):
+        if self.h:
+            abi.load().tsdf_batch_graph_destroy(self.h)
+            self.h = None
+
+
+def serve(engine: Engine, maps, requests, points: DeviceBuffer, max_batch_scans=32, max_inflight=2,
+          max_batch_points=1 << 22, deadline_ms=0.0):
+    """Runs the continuous-batching loop.  requests: (arrival_s, map_index,
+    n_points, point_offset, sensor), sorted by arrival.  Returns
+    (latency_ms array, batch index array, stats dict)."""
+    n = len(requests)
+    arr = (abi.Request * max(n, 1))()
+    for i, (t, mi, npts, off, s) in enumerate(requests):
+        arr[i].arrival_s = float(t)
+        arr[i].map_index = int(mi)
+        arr[i].n_points = int(npts)
+        arr[i].point_offset = int(off)
+        arr[i].sensor[:] = [float(x) for x in s]
+    cfg = abi.ServeConfig(int(max_batch_scans), int(max_inflight), int(max_batch_points), float(deadline_ms))
+    lat = np.zeros(max(n, 1), np.float64)
+    bof = np.zeros(max(n, 1), np.int32)
+    st = abi.ServeStats()
+    mp = (C.c_void_p * len(maps))(*[m.h for m in maps])
+    check(abi.load().tsdf_serve_run(engine.h, mp, len(maps), arr, n, C.c_void_p(points.ptr), C.byref(cfg),
+                                     lat.ctypes.data_as(C.POINTER(C.c_double)), bof.ctypes.data_as(abi.i32p),
+                                     C.byref(st)))
+    stats = {f: getattr(st, f) for f, _ in abi.ServeStats._fields_}
+    return lat[:n], bof[:n], stats

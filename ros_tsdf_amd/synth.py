@@ -85,22 +85,39 @@ def sensor_pose(spec: SceneSpec, k: int, n_scans: int, phase: float = 0.0) -> np
     return np.array([spec.radius * np.cos(th), spec.radius * np.sin(th), spec.sensor_z], dtype=np.float64)
 
 
+def _wrap(a):
+    return (a + np.pi) % (2.0 * np.pi) - np.pi
+
+
 def ray_cast(origin: np.ndarray, dirs: np.ndarray, boxes: np.ndarray) -> np.ndarray:
-    """Distance along unit rays to the first surface (inf for a miss)."""
+    """Distance along unit rays to the first surface (inf for a miss).
+
+    Rays are culled per box by the azimuth interval its footprint subtends
+    (the sensor is never inside a footprint), then slab-tested."""
     t = np.full(dirs.shape[0], np.inf)
+    az = np.arctan2(dirs[:, 1], dirs[:, 0])
     with np.errstate(divide="ignore", invalid="ignore"):
         dz = dirs[:, 2]
         tp = (GROUND_Z - origin[2]) / dz
         ok = (dz < 0) & (tp > 0)
         t = np.where(ok, tp, t)
-        inv = 1.0 / dirs
         for b in boxes:
+            cx, cy = 0.5 * (b[0] + b[3]) - origin[0], 0.5 * (b[1] + b[4]) - origin[1]
+            c = np.arctan2(cy, cx)
+            corners = np.array([[b[0], b[1]], [b[0], b[4]], [b[3], b[1]], [b[3], b[4]]]) - origin[:2]
+            dl = _wrap(np.arctan2(corners[:, 1], corners[:, 0]) - c)
+            rel = _wrap(az - c)
+            idx = np.nonzero((rel >= dl.min() - 1e-9) & (rel <= dl.max() + 1e-9))[0]
+            if idx.size == 0:
+                continue
+            d = dirs[idx]
+            inv = 1.0 / d
             t1 = (b[:3] - origin) * inv
             t2 = (b[3:] - origin) * inv
             tn = np.nanmax(np.minimum(t1, t2), axis=1)
             tf = np.nanmin(np.maximum(t1, t2), axis=1)
-            hit = (tn <= tf) & (tn > 0)
-            t = np.where(hit & (tn < t), tn, t)
+            hit = (tn <= tf) & (tn > 0) & (tn < t[idx])
+            t[idx[hit]] = tn[hit]
     return t
 
 

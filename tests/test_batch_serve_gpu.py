@@ -1,0 +1,168 @@
+"""GPU: batched pass, HIP-graph replays, serving loop and FP16 state.
+
+* a batch of scans from several maps gives bit-identical maps to the per-scan
+  path (same kernels, same per-voxel fold order);
+* graph replays match the eager batch path on every replay, with eager work
+  and synchronisations between replays;
+* a served map ends bit-identical whether its scans ran alone or mixed with
+  other maps' scans, and matches the oracle within tolerance;
+* FP16 state keeps keys and touched sets exact, SDF within the packed tolerance.
+"""
+import numpy as np
+import pytest
+
+from ros_tsdf_amd import abi, configs, synth, workloads
+from tests.helpers import compare_maps
+
+pytestmark = pytest.mark.gpu
+
+P = configs.SMALL_PARAMS
+
+
+def small_scans(n_maps, per_map, n_points=None):
+    out = []
+    for mi in range(n_maps):
+        w = configs.map_workload(configs.SMALL, mi)
+        boxes = synth.make_boxes(w.scene, w.seed, mi)
+        for k in range(per_map):
+            p, s, _ = synth.make_scan(w, k, map_id=mi, boxes=boxes, n_points=n_points)
+            out.append((p, s, mi))
+    return out
+
+
+def exports_equal(a, b):
+    ka, sa, wa = a.export()
+    kb, sb, wb = b.export()
+    return np.array_equal(ka, kb) and np.array_equal(sa.view(np.uint32), sb.view(np.uint32)) and \
+        np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from ros_tsdf_amd import Engine
+    e = Engine(heap_blocks=1 << 16, total_buckets=1 << 18, max_batch_points=1 << 16)
+    yield e
+    e.close()
+
+
+def test_batch_equals_per_scan(eng):
+    scans = small_scans(3, 4)
+    # interleave maps: scan order within each map preserved
+    order = sorted(range(len(scans)), key=lambda i: (i % 4, scans[i][2]))
+    scans = [scans[i] for i in order]
+    pool = workloads.Pool(scans)
+    buf = eng.buffer(pool.points.nbytes)
+    buf.upload(pool.points)
+    mb = [eng.map(P, buckets_log2=12) for _ in range(3)]
+    ms = [eng.map(P, buckets_log2=12) for _ in range(3)]
+    descs = [(pool.maps[i], pool.counts[i], pool.offsets[i], pool.sensors[i]) for i in range(len(pool))]
+    eng.integrate_batch(mb, descs, buf)
+    for (mi, n, off, s) in descs:
+        ms[mi].integrate(buf, n, s, offset_points=off)
+    eng.synchronize()
+    for a, b in zip(mb, ms):
+        assert exports_equal(a, b)
+    for m in mb + ms:
+        m.close()
+    buf.free()
+
+
+def test_graph_replays_match_eager(eng):
+    from ros_tsdf_amd import BatchGraph
+    scans = small_scans(2, 6)
+    pool = workloads.Pool(scans)
+    buf = eng.buffer(pool.points.nbytes)
+    buf.upload(pool.points)
+    batches = [[0, 6, 1, 7], [2, 8, 3, 9], [4, 10, 5, 11]]  # 2 scans of each map per batch, in map order
+    mg = [eng.map(P, buckets_log2=12) for _ in range(2)]
+    me = [eng.map(P, buckets_log2=12) for _ in range(2)]
+    g = BatchGraph(eng, mg, max_points=4 * 1000, max_scans=4)
+    # batches must be contiguous in the point buffer: build one buffer per batch
+    bufs = []
+    for b in batches:
+        sub = workloads.Pool([scans[i] for i in b])
+        bb = eng.buffer(sub.points.nbytes)
+        bb.upload(sub.points)
+        bufs.append((sub, bb))
+    for r, (sub, bb) in enumerate(bufs):
+        d = [(sub.maps[i], sub.counts[i], sub.offsets[i], sub.sensors[i]) for i in range(len(sub))]
+        g.launch(d, bb)
+        eng.synchronize()
+        eng.integrate_batch(me, d, bb)  # eager work between replays
+        eng.synchronize()
+        for a, b in zip(mg, me):
+            assert exports_equal(a, b), "replay %d differs from the eager pass" % r
+    g.close()
+    for m in mg + me:
+        m.close()
+    for _, bb in bufs:
+        bb.free()
+    buf.free()
+
+
+def test_serving_alone_vs_mixed(eng, oracle_lib):
+    from ros_tsdf_amd import serve
+    n_maps, per_map = 4, 6
+    scans = small_scans(n_maps, per_map)
+    pool = workloads.Pool(scans)
+    buf = eng.buffer(pool.points.nbytes)
+    buf.upload(pool.points)
+    rng = np.random.default_rng(3)
+    # requests: each map's scans in order, maps interleaved, bursty arrivals
+    reqs = []
+    nxt = [0] * n_maps
+    while any(x < per_map for x in nxt):
+        mi = int(rng.integers(0, n_maps))
+        if nxt[mi] >= per_map:
+            continue
+        j = mi * per_map + nxt[mi]
+        nxt[mi] += 1
+        t = 0.002 * (len(reqs) // 6)  # bursts of 6 simultaneous arrivals
+        reqs.append((t, mi, pool.counts[j], pool.offsets[j], pool.sensors[j]))
+    maps = [eng.map(P, buckets_log2=12) for _ in range(n_maps)]
+    lat, bof, st = serve(eng, maps, reqs, buf, max_batch_scans=8, max_inflight=2)
+    assert st["requests"] == len(reqs) and np.all(lat > 0)
+    assert st["batches"] < len(reqs), "requests were never batched together"
+    # map 0 alone, one request at a time
+    alone = [eng.map(P, buckets_log2=12)]
+    reqs0 = [(t, 0, n, off, s) for (t, mi, n, off, s) in reqs if mi == 0]
+    serve(eng, alone, reqs0, buf, max_batch_scans=1, max_inflight=1)
+    assert exports_equal(maps[0], alone[0])
+    # and the oracle
+    om = oracle_lib.OracleMap(P)
+    for k in range(per_map):
+        om.integrate(scans[k][0], scans[k][1])
+    compare_maps(maps[0], om, P[1], configs.SDF_TOL_T, configs.WEIGHT_TOL_REL * P[2])
+    om.close()
+    for m in maps + alone:
+        m.close()
+    buf.free()
+
+
+def test_fp16_state_error(oracle_lib):
+    from ros_tsdf_amd import Engine
+    params = configs.DENSE_PARAMS
+    with Engine(heap_blocks=1 << 16, total_buckets=1 << 18, max_batch_points=1 << 17,
+                state_format=abi.TSDF_STATE_F16) as e16:
+        m = e16.map(params, buckets_log2=15)
+        om = oracle_lib.OracleMap(params)
+        buf = e16.buffer(131072 * 16)
+        for k, pts, s in synth.scans(configs.DENSE, stop=2):
+            buf.upload(pts)
+            m.integrate(buf, pts.shape[0], s)
+            om.integrate(pts, s, threads=8)
+        # FP16 sdf/T has a half-ulp of 2^-11 on [0.5, 1): tolerance 1e-3 T;
+        # FP16 weights carry 11 significant bits: tolerance 1 % of the weight
+        gk, gs, gw = m.export()
+        ok, os_, ow = om.export()
+        assert np.array_equal(gk, ok)
+        t = ow != 0
+        assert np.array_equal(gw != 0, t)
+        e = np.abs(gs[t] - os_[t]) / params[1]
+        we = np.abs(gw[t] - ow[t]) / ow[t]
+        print("fp16 max %.3e rms %.3e T; weight rel max %.3e" % (e.max(), np.sqrt(np.mean(e * e)), we.max()))
+        assert e.max() <= 1e-3
+        assert we.max() <= 1e-2
+        buf.free()
+        m.close()
+        om.close()
