@@ -130,7 +130,8 @@ def main():
     if os.path.exists(tpath):
         try:
             with open(tpath) as f:
-                traffic = json.load(f).get(dom_kernel)
+                tj = json.load(f)
+            traffic = next((v for k, v in tj.items() if k.split("<")[0] == dom_kernel), None)
         except Exception:
             traffic = None
     roofline = {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
@@ -141,6 +142,16 @@ def main():
     result["stages_ms"] = {k: round(v, 5) for k, v in stage_ms.items()}
     result["per_scan"] = {"points": float(np.mean(N)), "voxel_updates": float(np.mean(M)),
                           "unique_voxels": float(np.mean(U)), "blocks_after": m.num_blocks()}
+    # a plain 256 MiB device copy: the known-size reference for the HBM byte counters
+    cb = 256 << 20
+    csrc, cdst = eng.buffer(cb), eng.buffer(cb)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        abi.check(abi.load().tsdf_device_copy(eng.h, abi.C.c_void_p(cdst.ptr), abi.C.c_void_p(csrc.ptr), cb))
+    copy_gbps = 3 * 2 * cb / (time.perf_counter() - t0) / 1e9
+    roofline["copy_kernel_GBps"] = copy_gbps
+    csrc.free()
+    cdst.free()
     plain_lat = None
     if extras:
         # ------------- full per-scan call latency (reference sequence) -------------

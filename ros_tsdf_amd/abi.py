@@ -116,6 +116,7 @@ SIGNATURES = {
     "tsdf_device_free": [vp, vp],
     "tsdf_copy_to_device": [vp, vp, vp, C.c_int64],
     "tsdf_copy_to_host": [vp, vp, vp, C.c_int64],
+    "tsdf_device_copy": [vp, vp, vp, C.c_int64],
     "tsdf_derived_params": [C.POINTER(Params), fp, i32p],
 }
 SPECIAL_RESTYPE = {"tsdf_last_error": C.c_char_p, "tsdf_engine_stream": vp}

@@ -775,6 +775,12 @@ __global__ void k_gc_orphans(Dev D, int map_slot) {
   if (threadIdx.x == 0) mc->orphan_n = 0;
 }
 
+// ---- plain device copy (also the reference point for HBM byte counters) ------------
+__global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // ---- initialisation ----------------------------------------------------------------------
 __global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;

@@ -1140,6 +1140,18 @@ int tsdf_copy_to_host(tsdf_engine* E, void* dst, const void* src, int64_t bytes)
   return TSDF_OK;
 }
 
+int tsdf_device_copy(tsdf_engine* E, void* dst, const void* src, int64_t bytes) {
+  if (!E || bytes < 0 || (bytes % 16) != 0 || (bytes > 0 && (!dst || !src)))
+    return fail(TSDF_ERR_INVALID_ARG, "bad argument (bytes must be a multiple of 16)");
+  CHK(set_device(E));
+  const int64_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256, 8192)), dim3(256), 0, E->stream, reinterpret_cast<const float4*>(src),
+                     reinterpret_cast<float4*>(dst), n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(E->stream));
+  return TSDF_OK;
+}
+
 // ---- serving ---------------------------------------------------------------------------
 int tsdf_serve_run(tsdf_engine* E, tsdf_map* const* maps, int32_t nmaps, const tsdf_request* reqs, int32_t nreq,
                    const float* points_dev, const tsdf_serve_config* cfg, double* latency_ms, int32_t* batch_of,
