@@ -126,7 +126,8 @@ _lib = None
 
 
 def library_path() -> str:
-    return _build.ENGINE_SO
+    """The built engine library; TSDF_HIP_LIBRARY overrides it (for comparing builds)."""
+    return os.environ.get("TSDF_HIP_LIBRARY", _build.ENGINE_SO)
 
 
 def load():

@@ -15,6 +15,7 @@ constexpr int kBucket = 8;             // entries per bucket (one 64-byte line o
 constexpr int kMaxScansPerBatch = 256;
 constexpr int kPointIdxBits = 23;      // point index inside one pass
 constexpr int kMaxPassPoints = (1 << kPointIdxBits) - 2;
+constexpr int kPartSlots = 1024;       // statistics are summed into this many slots, then reduced
 
 // Statistic slots (EngCtr::stats), mirrored by tsdf_scan_stats.
 enum Stat {
@@ -27,6 +28,7 @@ enum Stat {
 struct MapDev {
   float vs, T, maxW, pubD2, gcD2;
   float half_vs, BL, half_BL;
+  float sc_v, sc_b;       // 1/voxel_size and 1/block_size, as getVolumeCenterFromPoint computes them
   int32_t max_vox, max_blk;
   uint32_t home_mask;     // home buckets - 1 (power of two)
   int32_t bucket_base;    // global index of home bucket 0
@@ -51,8 +53,10 @@ struct EngCtr {
   int32_t new_n;      // entries inserted in this pass
   int32_t sel_n;      // selected blocks (publish / export)
   int32_t freed_n;    // blocks freed by the current gc
-  int32_t big_n;      // blocks whose update list exceeds one LDS tile
+  int32_t big_n;      // blocks whose update list exceeds one wave's LDS
   int32_t pad[2];
+  int32_t fold_n[6];  // touched blocks per fold size class
+  int32_t pad2[2];
   unsigned long long stats[ST_COUNT];
 };
 
@@ -79,7 +83,8 @@ struct Dev {
   EngCtr* ctr;
   unsigned long long* keys;  // [entries]
   int32_t* vals;             // [entries] heap block of an entry, -1 none
-  uint32_t* blk_cnt;         // [entries] updates routed to the entry in this pass
+  unsigned long long* blk_cnt;  // [entries] pass counters: (point, block) pairs in the low
+                                //   32 bits, voxel records in the high 32 bits
   uint32_t* touch_idx;       // [entries] index in the touched list
   int32_t* next;             // [buckets] overflow link, -1 none
   int32_t* ovf_stack;        // [buckets] per-map overflow free stacks
@@ -90,17 +95,21 @@ struct Dev {
   // pass scratch
   const PassHdr* hdr;
   const ScanDev* scans;
-  uint32_t* rec_e;           // [stride][pcap] entry of each update record
-  unsigned long long* rec_kd;  // [stride][pcap] (local << 55 | point << 32 | sdf bits)
-  uint8_t* rec_n;            // [pcap]
+  uint32_t* pair_e;          // [stride][pcap] block entry of each (point, block) pair
+  uint32_t* pair_r;          // [stride][pcap] record offset of the run inside its block
+  uint8_t* pair_n;           // [pcap] pairs of each point
   int64_t pcap;
   uint32_t* touched;         // [entries]
-  uint32_t* seg_off;         // [entries + 1]
+  uint8_t* touched_map;      // [entries] map slot of each touched block
+  uint32_t* seg_off;         // [entries + 1] record offset of each touched block
   uint32_t* tile_sums;       // [entries / 1024 + 2]
   uint32_t* new_list;        // [entries]
   int32_t* new_map;          // [entries]
   uint32_t* big_list;        // [entries]
-  unsigned long long* seg;   // [records]
+  int4* fold_desc;           // [6][fold_cap] per size class: {record base, records, heap block, map}
+  unsigned long long* part;  // [ST_COUNT][kPartSlots] statistics partial sums
+  int64_t fold_cap;
+  unsigned long long* seg;   // [records] update records, contiguous per touched block
   int64_t seg_cap;
   uint32_t* sel;             // [entries]
   uint32_t* pub_cnt;         // [entries + 1]

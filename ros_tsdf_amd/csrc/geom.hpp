@@ -81,43 +81,45 @@ __device__ __forceinline__ void truncation_segment(const float p[3], const float
   }
 }
 
-// traverseVolume (H:146-245): steps a point from a towards b in increments of
-// +-L, one or more axes at a time by the reference's tMax comparison tree,
-// re-deriving the cell centre after each step; stops when the centre matches
-// b's cell within L/4 (emitting it), or without emitting when a step moved no
-// axis by a full cell or some axis by more than one (the reference's precision
-// guard).  At most max_cells cells are emitted; *trunc reports the cut.
-// emit(k0, k1, k2, c0, c1, c2) receives each cell's integer key and centre.
+// traverseVolume (H:146-245) for the cell keys: steps a point from a towards
+// b in increments of +-L, one or more axes at a time by the reference's tMax
+// comparison tree, and follows the integer cell k = floor(x * s), s = 1/L.
+// The reference compares cell centres with tolerance L/4: "centres equal"
+// (loop end, |c - c_b| < L/4 on every axis) and its precision guard (a step
+// that moved no axis by a full cell, or some axis by more than one) are
+// tests on k alone, because centres of adjacent cells are L apart up to FP32
+// rounding that the point validity rule keeps far below L/4.  Only the setup
+// (tMax from the centre of a's cell, tDelta) needs FP32 divisions.
+// At most max_cells cells are emitted; trunc reports the cut.
+// emit(k0, k1, k2) receives each cell and returns false to stop the walk
+// early (the caller has seen every cell it needs).
 template <class Emit>
-__device__ __forceinline__ int traverse(const float a[3], const float b[3], float L, int max_cells, bool& trunc,
-                                        Emit&& emit) {
+__device__ __forceinline__ int walk_cells(const float a[3], const float b[3], float L, float scale, float half,
+                                          int max_cells, bool& trunc, Emit&& emit) {
   trunc = false;
   const float v[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-  const float half = L / 2.0f;
-  const float eps = L / 4.0f;
-  const float Lpe = L + eps;
-  const float Lme = L - eps;
-  float step[3], tMax[3], tDelta[3], cur[3], cc[3], ce[3];
-  int32_t kc[3], ke[3];
+  float step[3], tMax[3], tDelta[3], cur[3];
+  int32_t k[3], ke[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     step[i] = v[i] > 0 ? L : -1.0f * L;
-    int32_t kd;
-    const float c0 = cell_center(a[i], L, kd);
+    const float f = floorf(a[i] * scale);
+    const float c0 = f / scale + half;
     tMax[i] = fabsf(v[i] < 0 ? (c0 - half - a[i]) / v[i] : (c0 + half - a[i]) / v[i]);
     tDelta[i] = fabsf(L / v[i]);
     cur[i] = a[i];
-    cc[i] = cell_center(cur[i], L, kc[i]);
-    ce[i] = cell_center(b[i], L, ke[i]);
+    k[i] = (int32_t)f;
+    ke[i] = (int32_t)floorf(b[i] * scale);
   }
   int n = 0;
-  while (!(fabsf(cc[0] - ce[0]) < eps && fabsf(cc[1] - ce[1]) < eps && fabsf(cc[2] - ce[2]) < eps)) {
+  while (!(k[0] == ke[0] && k[1] == ke[1] && k[2] == ke[2])) {
     if (n == max_cells) {
       trunc = true;
       return n;
     }
-    emit(kc[0], kc[1], kc[2], cc[0], cc[1], cc[2]);
+    const bool go = emit(k[0], k[1], k[2]);
     ++n;
+    if (!go) return n;
     bool sx = false, sy = false, sz = false;
     if (tMax[0] < tMax[1]) {
       if (tMax[0] < tMax[2]) sx = true;
@@ -135,41 +137,37 @@ __device__ __forceinline__ int traverse(const float a[3], const float b[3], floa
     if (sx) { cur[0] += step[0]; tMax[0] += tDelta[0]; }
     if (sy) { cur[1] += step[1]; tMax[1] += tDelta[1]; }
     if (sz) { cur[2] += step[2]; tMax[2] += tDelta[2]; }
-    float pc[3] = {cc[0], cc[1], cc[2]};
+    int32_t kn[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) cc[i] = cell_center(cur[i], L, kc[i]);
-    const float d0 = fabsf(pc[0] - cc[0]);
-    const float d1 = fabsf(pc[1] - cc[1]);
-    const float d2 = fabsf(pc[2] - cc[2]);
-    if ((d0 < Lme && d1 < Lme && d2 < Lme) || (d0 > Lpe || d1 > Lpe || d2 > Lpe)) return n;
+    for (int i = 0; i < 3; ++i) kn[i] = (int32_t)floorf(cur[i] * scale);
+    const int32_t d0 = abs(kn[0] - k[0]), d1 = abs(kn[1] - k[1]), d2 = abs(kn[2] - k[2]);
+    if ((d0 == 0 && d1 == 0 && d2 == 0) || d0 > 1 || d1 > 1 || d2 > 1) return n;
+    k[0] = kn[0];
+    k[1] = kn[1];
+    k[2] = kn[2];
   }
   if (n == max_cells) {
     trunc = true;
     return n;
   }
-  emit(kc[0], kc[1], kc[2], cc[0], cc[1], cc[2]);
+  emit(k[0], k[1], k[2]);
   return n + 1;
 }
 
-// Voxel centre -> block key and local voxel index: getVolumeCenterFromPoint
-// with the block size (H:612), bottom-left corner (H:622-624) and
-// getLocalVoxelIndex (H:507-512).  Returns false when a floor leaves 0..7.
-__device__ __forceinline__ bool voxel_to_block(const float vc[3], float vs, float BL, float half_BL, int32_t kb[3],
-                                               int& local) {
-  float f[3];
-  bool ok = true;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const float bc = cell_center(vc[i], BL, kb[i]);
-    const float bl = bc - half_BL;
-    float d = vc[i] - bl;
-    d = d / vs;
-    f[i] = floorf(d);
-    ok = ok && (f[i] >= 0.0f) && (f[i] <= 7.0f);
-  }
-  const float idx = f[0] + (f[1] * (float)kVoxelsPerSide) + (f[2] * (float)kVoxelsPerSide * (float)kVoxelsPerSide);
-  local = (int)idx;
-  return ok;
+// Centre of cell k: getVolumeCenterFromPoint (H:76-85) with floor(x * s)
+// replaced by the k it produced.
+__device__ __forceinline__ float cell_centre_of(int32_t k, float scale, float half) {
+  return (float)k / scale + half;
+}
+
+// Block and local index of voxel k.  The reference derives them from the
+// voxel centre in FP32 (H:612, H:622-626, H:507-512): floor(c / (8 L)) and
+// floor((c - block corner) / L).  With centres (k + 1/2) L up to rounding,
+// those floors are floor(k / 8) and k mod 8 exactly (the fractional parts are
+// at least 1/16 away from an integer, far above FP32 error at valid points).
+__device__ __forceinline__ int32_t block_of(int32_t k) { return k >> 3; }
+__device__ __forceinline__ int local_index(const int32_t k[3]) {
+  return (k[0] & 7) + 8 * (k[1] & 7) + 64 * (k[2] & 7);
 }
 
 // getDistanceUpdate (H:562-573): |p - x|, negated when (p - s).(p - x) < 0.

@@ -100,7 +100,7 @@ struct tsdf_engine {
   EngCtr* ctr_host = nullptr;  // pinned
   unsigned long long* keys = nullptr;
   int32_t* vals = nullptr;
-  uint32_t* blk_cnt = nullptr;
+  unsigned long long* blk_cnt = nullptr;
   uint32_t* touch_idx = nullptr;
   int32_t* next = nullptr;
   int32_t* ovf_stack = nullptr;
@@ -116,9 +116,9 @@ struct tsdf_engine {
   hipEvent_t stage_ev[kRing] = {};
   bool stage_used[kRing] = {};
   int stage_next = 0;
-  uint32_t* rec_e = nullptr;
-  unsigned long long* rec_kd = nullptr;
-  uint8_t* rec_n = nullptr;
+  uint32_t* pair_e = nullptr;
+  uint32_t* pair_r = nullptr;
+  uint8_t* pair_n = nullptr;
   int64_t pcap = 0;
   int stride_cap = 0;
   unsigned long long* seg = nullptr;
@@ -129,6 +129,10 @@ struct tsdf_engine {
   uint32_t* new_list = nullptr;
   int32_t* new_map = nullptr;
   uint32_t* big_list = nullptr;
+  int4* fold_desc = nullptr;
+  unsigned long long* part = nullptr;
+  int64_t fold_cap = 0;
+  uint8_t* touched_map = nullptr;
   uint32_t* sel = nullptr;
   uint32_t* pub_cnt = nullptr;
   uint32_t* pub_off = nullptr;
@@ -178,9 +182,9 @@ struct tsdf_engine {
     D.freed = freed;
     D.hdr = hdr_dev;
     D.scans = scans_dev;
-    D.rec_e = rec_e;
-    D.rec_kd = rec_kd;
-    D.rec_n = rec_n;
+    D.pair_e = pair_e;
+    D.pair_r = pair_r;
+    D.pair_n = pair_n;
     D.pcap = pcap;
     D.touched = touched;
     D.seg_off = seg_off;
@@ -188,6 +192,10 @@ struct tsdf_engine {
     D.new_list = new_list;
     D.new_map = new_map;
     D.big_list = big_list;
+    D.fold_desc = fold_desc;
+    D.part = part;
+    D.fold_cap = fold_cap;
+    D.touched_map = touched_map;
     D.seg = seg;
     D.seg_cap = seg_cap;
     D.sel = sel;
@@ -224,6 +232,8 @@ void derive(const tsdf_params& p, MapDev& m) {
   m.half_vs = p.voxel_size / 2;
   m.BL = p.voxel_size * kVoxelsPerSide;
   m.half_BL = m.BL / 2;
+  m.sc_v = 1 / m.vs;
+  m.sc_b = 1 / m.BL;
   m.max_vox = (int)(std::ceil(p.truncation_distance * 2 / p.voxel_size) * 4);
   m.max_blk = (int)(std::ceil(p.truncation_distance * 2 / m.BL) * 4);
 }
@@ -259,20 +269,24 @@ int ensure_scratch(tsdf_engine* E, int64_t P, int stride) {
   int64_t np = std::max<int64_t>(P, E->pcap);
   np = std::max<int64_t>(np, 1024);
   int ns = std::max(stride, E->stride_cap);
-  hipFree(E->rec_e);
-  hipFree(E->rec_kd);
-  hipFree(E->rec_n);
-  hipFree(E->seg);
-  E->rec_e = nullptr;
-  E->rec_kd = nullptr;
-  E->rec_n = nullptr;
+  (void)hipFree(E->pair_e);
+  (void)hipFree(E->pair_r);
+  (void)hipFree(E->pair_n);
+  (void)hipFree(E->seg);
+  E->pair_e = nullptr;
+  E->pair_r = nullptr;
+  E->pair_n = nullptr;
   E->seg = nullptr;
   E->pcap = 0;
   E->stride_cap = 0;
-  CHK(dev_alloc(&E->rec_e, np * ns));
-  CHK(dev_alloc(&E->rec_kd, np * ns));
-  CHK(dev_alloc(&E->rec_n, np));
+  CHK(dev_alloc(&E->pair_e, np * ns));
+  CHK(dev_alloc(&E->pair_r, np * ns));
+  CHK(dev_alloc(&E->pair_n, np));
   CHK(dev_alloc(&E->seg, np * ns));
+  (void)hipFree(E->fold_desc);
+  E->fold_desc = nullptr;
+  E->fold_cap = std::min<int64_t>(E->entries, np * ns);
+  CHK(dev_alloc(&E->fold_desc, 6 * E->fold_cap));
   E->pcap = np;
   E->stride_cap = ns;
   E->seg_cap = np * ns;
@@ -349,31 +363,43 @@ int enqueue_pass(tsdf_engine* E, int64_t P, int tid) {
   Dev D = E->view();
   hipStream_t st = E->stream;
   const tsdf_launch_config& L = E->launch;
+  const uint32_t pass_stats = ((1u << ST_COUNT) - 1u) & ~((1u << ST_PUBLISHED) | (1u << ST_GC_REMOVED));
   hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(64), 0, st, D);
+  hipLaunchKernelGGL(k_stats_zero, dim3(64), dim3(256), 0, st, D, pass_stats);
   CHK(timing_mark(E, tid, 0));
   const int tb = L.traverse_block;
-  hipLaunchKernelGGL(k_traverse, dim3(grid_for(P, tb, L.traverse_grid_cap)), dim3(tb), 0, st, D);
+  hipLaunchKernelGGL(k_bin, dim3(grid_for(P, tb, L.traverse_grid_cap)), dim3(tb), 0, st, D);
   CHK(timing_mark(E, tid, 1));
   hipLaunchKernelGGL(k_assign, dim3(grid_for(P, 256, 1024)), dim3(256), 0, st, D);
   CHK(timing_mark(E, tid, 2));
   const int64_t tmax = std::min<int64_t>(E->entries, P * E->stride_cap);
   const int sg = grid_for(tmax, kScanTile, 1024);
-  CountTouched ct{E->touched, E->blk_cnt};
-  hipLaunchKernelGGL(k_scan_tiles<CountTouched>, dim3(sg), dim3(256), 0, st, &E->ctr->touched_n, ct, E->tile_sums);
+  CountTouchedRecords ct{E->touched, E->blk_cnt};
+  hipLaunchKernelGGL(k_scan_tiles<CountTouchedRecords>, dim3(sg), dim3(256), 0, st, &E->ctr->touched_n, ct, E->tile_sums);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, &E->ctr->touched_n, E->tile_sums);
-  hipLaunchKernelGGL(k_scan_apply<CountTouched>, dim3(sg), dim3(256), 0, st, &E->ctr->touched_n, ct, E->tile_sums,
+  hipLaunchKernelGGL(k_scan_apply<CountTouchedRecords>, dim3(sg), dim3(256), 0, st, &E->ctr->touched_n, ct, E->tile_sums,
                      E->seg_off);
   CHK(timing_mark(E, tid, 3));
-  hipLaunchKernelGGL(k_scatter, dim3(grid_for(P, 256, 65536)), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k_emit, dim3(grid_for(P, 256, 65536)), dim3(256), 0, st, D);
   CHK(timing_mark(E, tid, 4));
   const int fg = std::max(1, L.fold_grid);
+  hipLaunchKernelGGL(k_fold_prep, dim3(grid_for(tmax, 256, 2048)), dim3(256), 0, st, D);
   if (E->state_format == TSDF_STATE_F16) {
-    hipLaunchKernelGGL(k_fold<StateF16>, dim3(fg), dim3(kFoldThreads), 0, st, D);
-    hipLaunchKernelGGL(k_fold_big<StateF16>, dim3(64), dim3(kFoldThreads), 0, st, D);
+    hipLaunchKernelGGL((k_fold_small<StateF16, 16>), dim3(fg), dim3(256), 0, st, D);
+    hipLaunchKernelGGL((k_fold_small<StateF16, 32>), dim3(fg), dim3(256), 0, st, D);
+    hipLaunchKernelGGL((k_fold_small<StateF16, 64>), dim3(fg), dim3(256), 0, st, D);
+    hipLaunchKernelGGL((k_fold_count<StateF16, kCountTileS, FC_WG>), dim3(2048), dim3(256), 0, st, D);
+    hipLaunchKernelGGL((k_fold_count<StateF16, kCountTile, FC_WG2>), dim3(512), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_fold_huge<StateF16>, dim3(64), dim3(kFoldThreads), 0, st, D);
   } else {
-    hipLaunchKernelGGL(k_fold<StateF32>, dim3(fg), dim3(kFoldThreads), 0, st, D);
-    hipLaunchKernelGGL(k_fold_big<StateF32>, dim3(64), dim3(kFoldThreads), 0, st, D);
+    hipLaunchKernelGGL((k_fold_small<StateF32, 16>), dim3(fg), dim3(256), 0, st, D);
+    hipLaunchKernelGGL((k_fold_small<StateF32, 32>), dim3(fg), dim3(256), 0, st, D);
+    hipLaunchKernelGGL((k_fold_small<StateF32, 64>), dim3(fg), dim3(256), 0, st, D);
+    hipLaunchKernelGGL((k_fold_count<StateF32, kCountTileS, FC_WG>), dim3(2048), dim3(256), 0, st, D);
+    hipLaunchKernelGGL((k_fold_count<StateF32, kCountTile, FC_WG2>), dim3(512), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_fold_huge<StateF32>, dim3(64), dim3(kFoldThreads), 0, st, D);
   }
+  hipLaunchKernelGGL(k_stats_reduce, dim3(ST_COUNT), dim3(256), 0, st, D, pass_stats);
   CHK(timing_mark(E, tid, 5));
   HIPCHK(hipGetLastError());
   return TSDF_OK;
@@ -522,12 +548,14 @@ int gc_impl(tsdf_map* m, const float sensor[3], int mode, int64_t* removed) {
   CHK(timing_begin(E, 2, &tid));
   CHK(timing_mark(E, tid, 0));
   hipLaunchKernelGGL(k_begin_select, dim3(1), dim3(1), 0, st, E->ctr, (int)ST_GC_REMOVED);
+  hipLaunchKernelGGL(k_stats_zero, dim3(4), dim3(256), 0, st, D, 1u << ST_GC_REMOVED);
   const int64_t ne = (int64_t)(m->desc.n_home + m->desc.n_ovf) * kBucket;
   hipLaunchKernelGGL(k_gc_mark, dim3(grid_for(ne, 256, 4096)), dim3(256), 0, st, D, m->slot, sensor[0], sensor[1],
                      sensor[2], mode);
   hipLaunchKernelGGL(k_gc_zero<State>, dim3(1024), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k_gc_chains, dim3(grid_for(m->desc.n_home, 256, 4096)), dim3(256), 0, st, D, m->slot);
   hipLaunchKernelGGL(k_gc_orphans, dim3(1), dim3(256), 0, st, D, m->slot);
+  hipLaunchKernelGGL(k_stats_reduce, dim3(ST_COUNT), dim3(256), 0, st, D, 1u << ST_GC_REMOVED);
   HIPCHK(hipGetLastError());
   CHK(timing_mark(E, tid, 1));
   timing_end(E, tid);
@@ -684,7 +712,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
       (r = dev_alloc(&E->hdr_dev, 1)) || (r = dev_alloc(&E->scans_dev, kMaxScansPerBatch)) ||
       (r = dev_alloc(&E->touched, entries)) || (r = dev_alloc(&E->seg_off, entries + 1)) ||
       (r = dev_alloc(&E->tile_sums, entries / kScanTile + 2)) || (r = dev_alloc(&E->new_list, entries)) ||
-      (r = dev_alloc(&E->new_map, entries)) || (r = dev_alloc(&E->big_list, entries)) ||
+      (r = dev_alloc(&E->new_map, entries)) || (r = dev_alloc(&E->big_list, entries)) || (r = dev_alloc(&E->touched_map, entries)) || (r = dev_alloc(&E->part, ST_COUNT * kPartSlots)) ||
       (r = dev_alloc(&E->sel, entries)) || (r = dev_alloc(&E->pub_cnt, entries + 1)) ||
       (r = dev_alloc(&E->pub_off, entries + 1)))
     return cleanup(r);
@@ -711,7 +739,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
   if (hipMemsetAsync(E->ctr, 0, sizeof(EngCtr), st) != hipSuccess ||
       hipMemsetAsync(E->mctr, 0, sizeof(MapCtr) * kMaxMaps, st) != hipSuccess ||
       hipMemsetAsync(E->heap, 0, (size_t)vbytes, st) != hipSuccess ||
-      hipMemsetAsync(E->blk_cnt, 0, sizeof(uint32_t) * entries, st) != hipSuccess ||
+      hipMemsetAsync(E->blk_cnt, 0, sizeof(unsigned long long) * entries, st) != hipSuccess ||
       hipMemsetAsync(E->keys, 0xFF, sizeof(unsigned long long) * entries, st) != hipSuccess ||
       hipMemsetAsync(E->vals, 0xFF, sizeof(int32_t) * entries, st) != hipSuccess ||
       hipMemsetAsync(E->next, 0xFF, sizeof(int32_t) * c.total_buckets, st) != hipSuccess)
@@ -739,8 +767,8 @@ int tsdf_engine_destroy(tsdf_engine* E) {
       E->map_by_slot[i] = nullptr;
     }
   void* ptrs[] = {E->maps_dev, E->mctr, E->ctr, E->keys, E->vals, E->blk_cnt, E->touch_idx, E->next, E->ovf_stack,
-                  E->orphans, E->heap, E->heap_free, E->freed, E->hdr_dev, E->scans_dev, E->rec_e, E->rec_kd, E->rec_n,
-                  E->seg, E->touched, E->seg_off, E->tile_sums, E->new_list, E->new_map, E->big_list, E->sel,
+                  E->orphans, E->heap, E->heap_free, E->freed, E->hdr_dev, E->scans_dev, E->pair_e, E->pair_r, E->pair_n,
+                  E->seg, E->touched, E->seg_off, E->tile_sums, E->new_list, E->new_map, E->big_list, E->fold_desc, E->part, E->touched_map, E->sel,
                   E->pub_cnt, E->pub_off, E->pub_buf};
   for (void* p : ptrs)
     if (p) hipFree(p);
