@@ -50,11 +50,10 @@ struct MapCtr {
 struct EngCtr {
   int32_t heap_top;   // entries on the heap free stack
   int32_t touched_n;  // blocks with updates in this pass
-  int32_t new_n;      // entries inserted in this pass
+  int32_t fail_n;     // entries inserted in this pass that got no heap block
   int32_t sel_n;      // selected blocks (publish / export)
   int32_t freed_n;    // blocks freed by the current gc
-  int32_t big_n;      // blocks whose update list exceeds one wave's LDS
-  int32_t pad[2];
+  int32_t pad[3];
   int32_t fold_n[6];  // touched blocks per fold size class
   int32_t pad2[2];
   unsigned long long stats[ST_COUNT];
@@ -83,8 +82,7 @@ struct Dev {
   EngCtr* ctr;
   unsigned long long* keys;  // [entries]
   int32_t* vals;             // [entries] heap block of an entry, -1 none
-  unsigned long long* blk_cnt;  // [entries] pass counters: (point, block) pairs in the low
-                                //   32 bits, voxel records in the high 32 bits
+  unsigned long long* blk_cnt;  // [entries] pass counter: voxel records in the high 32 bits
   uint32_t* touch_idx;       // [entries] index in the touched list
   int32_t* next;             // [buckets] overflow link, -1 none
   int32_t* ovf_stack;        // [buckets] per-map overflow free stacks
@@ -103,9 +101,7 @@ struct Dev {
   uint8_t* touched_map;      // [entries] map slot of each touched block
   uint32_t* seg_off;         // [entries + 1] record offset of each touched block
   uint32_t* tile_sums;       // [entries / 1024 + 2]
-  uint32_t* new_list;        // [entries]
-  int32_t* new_map;          // [entries]
-  uint32_t* big_list;        // [entries]
+  uint32_t* fail_list;       // [entries] inserted entries without a heap block (removed after binning)
   int4* fold_desc;           // [6][fold_cap] per size class: {record base, records, heap block, map}
   unsigned long long* part;  // [ST_COUNT][kPartSlots] statistics partial sums
   int64_t fold_cap;

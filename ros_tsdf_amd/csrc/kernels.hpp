@@ -1,24 +1,29 @@
 // Kernels of the integration pipeline.  One pass integrates the scans of a
 // batch (a single scan on the per-scan path):
 //
-//   k_pass_begin     zero the pass counters
+//   k_pass_begin     zero the pass counters and statistics partials
 //   k_bin            per point: truncation segment, block DDA and voxel DDA;
 //                    find-or-insert every block in the map's spatial hash
 //                    (lock-free) once per workgroup; each run of voxels in
-//                    one block gets its record offset in that block
-//   k_assign         pop a heap block from the free stack for each new entry
-//   k_scan_*         exclusive scan of the per-block record counts
+//                    one block gets its record offset in that block; the
+//                    workgroup's new entries get heap blocks from the
+//                    free stack (one pop per wave)
+//   k_scan_tiles     per-tile totals of the touched blocks' record counts
+//   k_seg_apply      record segment offset and 16-byte fold descriptor of
+//                    every touched block, filed by size class
 //   k_emit           per point: second voxel-DDA walk writing each update
 //                    record (voxel, point, sdf) to its block's contiguous
 //                    record segment at the offset reserved in k_bin
-//   k_fold_prep      one 16-byte descriptor per touched block, filed by size
-//   k_fold_small     blocks of <= 16 / 32 / 64 records, 4 / 2 / 1 per wave:
-//                    sort by (voxel, point) across lanes with shuffles; each
-//                    voxel's first record folds the voxel's updates in point
-//                    order (one read + one write per voxel)
-//   k_fold_count     65..4096 records, one workgroup: counting sort by voxel
-//                    in LDS, per-voxel insertion sort by point, fold
-//   k_fold_huge      larger blocks: sorted LDS tiles merged per voxel
+//   k_fold_a         blocks of 65..1024 records (a workgroup each: counting
+//                    sort by voxel in LDS, rank placement by point, fold),
+//                    then blocks of <= 64 / 32 / 16 records, 1 / 2 / 4 per
+//                    wave (bitonic sort across lanes with shuffles; each
+//                    voxel's first record folds the voxel's updates)
+//   k_fold_b         blocks of 1025..4096 records (counting sort), then
+//                    larger ones (sorted LDS tiles merged per voxel)
+// Every voxel's state is read once and written once per pass, its updates
+// folded in point order.  Statistics are summed into spread partial slots
+// and reduced on demand (k_stats_reduce).
 //
 // publish: k_select (radius test per entry) -> k_pub_count (wave per block)
 //          -> scan -> k_pub_write (wave ballot compaction)
@@ -34,6 +39,15 @@ constexpr int kCountTile = 4096;  // records per LDS tile of the large k_fold_co
 constexpr int kFoldCap = 8192;    // records per LDS sort tile of k_fold_huge (64 KB)
 constexpr int kMaxChunks = 32;    // sort tiles per block in k_fold_huge
 constexpr int kScanTile = 1024;  // elements per workgroup in the scan kernels
+
+// Size classes of touched blocks (records per block): the fold kernel and
+// the number of blocks folded together differ per class.
+enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WG, FC_WG2, FC_HUGE, FC_COUNT };
+static_assert(FC_COUNT <= (int)(sizeof(EngCtr::fold_n) / sizeof(int32_t)), "fold class counters");
+
+__host__ __device__ constexpr int fold_class(int n) {
+  return n <= 16 ? FC_16 : n <= 32 ? FC_32 : n <= 64 ? FC_64 : n <= kCountTileS ? FC_WG : n <= kCountTile ? FC_WG2 : FC_HUGE;
+}
 
 // ---- wave / workgroup helpers -------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -69,8 +83,8 @@ __device__ __forceinline__ void stat_add(const Dev& D, int stat, unsigned long l
   }
 }
 
-template <int N>
-__device__ __forceinline__ void flush_stats(const Dev& D, const int* which, const unsigned long long (&v)[N]) {
+template <class T, int N>
+__device__ __forceinline__ void flush_stats(const Dev& D, const int* which, const T (&v)[N]) {
 #pragma unroll
   for (int i = 0; i < N; ++i) stat_add(D, which[i], v[i]);
 }
@@ -184,14 +198,18 @@ __device__ inline int find_or_insert(const Dev& D, const MapDev& m, unsigned lon
 }
 
 // ---- pass kernels -----------------------------------------------------------------
-__global__ void k_pass_begin(Dev D) {
-  const int t = threadIdx.x;
-  if (t == 0) {
-    D.ctr->touched_n = 0;
-    D.ctr->new_n = 0;
-    D.ctr->big_n = 0;
+// Zeroes the pass counters and the partial slots of the statistics in mask.
+__global__ void __launch_bounds__(256) k_pass_begin(Dev D, uint32_t mask) {
+  if (blockIdx.x == 0) {
+    const int t = threadIdx.x;
+    if (t == 0) {
+      D.ctr->touched_n = 0;
+      D.ctr->fail_n = 0;
+    }
+    if (t < FC_COUNT) D.ctr->fold_n[t] = 0;
   }
-  if (t < 6) D.ctr->fold_n[t] = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ST_COUNT * kPartSlots; i += gridDim.x * blockDim.x)
+    if (mask & (1u << (i / kPartSlots))) D.part[i] = 0;
 }
 
 __device__ __forceinline__ int find_scan(const ScanDev* scans, int ns, int i) {
@@ -204,10 +222,93 @@ __device__ __forceinline__ int find_scan(const ScanDev* scans, int ns, int i) {
   return lo;
 }
 
-__device__ __forceinline__ void note_new(const Dev& D, int e, int map_slot) {
-  const int q = atomicAdd(&D.ctr->new_n, 1);
-  D.new_list[q] = (uint32_t)e;
-  D.new_map[q] = map_slot;
+// Heap blocks for new entries (free-list pop; pushes only happen in gc,
+// never concurrently with pops, so the stack needs no ABA guard).  All
+// lanes of the wave call this; lanes with active == false only take part in
+// the ballots.  Both counters are updated once per wave: the per-map block
+// count once per distinct map in the wave, the heap stack once; lanes past
+// a quota or past the bottom of the stack hand their share back.  An entry
+// that gets no block is filed on the fail list (its key leaves the table
+// after binning) and its vals[] is -1.
+// Returns kAssignNone for inactive lanes, else kAssignOk, kAssignHeapFail or
+// kAssignQuotaFail.
+constexpr int kAssignNone = -1, kAssignOk = 0, kAssignHeapFail = 1, kAssignQuotaFail = 2;
+__device__ __forceinline__ int assign_heap(const Dev& D, bool active, uint32_t e, int ms) {
+  const int lane = lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int held = 0;  // blocks the map held before this lane's block
+  unsigned long long pending = __ballot(active);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lm = __shfl(ms, leader);
+    const unsigned long long grp = __ballot(active && ms == lm);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&D.mctr[lm].blocks, (int)__popcll(grp));
+    base = __shfl(base, leader);
+    if (grp >> lane & 1ull) held = base + (int)__popcll(grp & below);
+    pending &= ~grp;
+  }
+  bool ok = active;
+  bool quota_fail = false;
+  if (active) {
+    const int64_t quota = D.maps[ms].block_quota;
+    if (quota > 0 && held >= quota) {
+      atomicSub(&D.mctr[ms].blocks, 1);
+      ok = false;
+      quota_fail = true;
+    }
+  }
+  const unsigned long long want = __ballot(ok);
+  const int cnt = (int)__popcll(want);
+  const int leader = want ? __ffsll((long long)want) - 1 : 0;
+  int top = 0;
+  if (want && lane == leader) {
+    top = atomicSub(&D.ctr->heap_top, cnt);
+    const int failed = min(cnt, cnt - max(top, 0));
+    if (failed > 0) atomicAdd(&D.ctr->heap_top, failed);
+  }
+  top = __shfl(top, leader);
+  const int idx = top - cnt + (int)__popcll(want & below);
+  if (ok && idx < 0) {
+    atomicSub(&D.mctr[ms].blocks, 1);
+    ok = false;
+  }
+  if (!active) return kAssignNone;
+  if (ok) {
+    D.vals[e] = D.heap_free[idx];
+    return kAssignOk;
+  }
+  D.vals[e] = -1;
+  D.fail_list[atomicAdd(&D.ctr->fail_n, 1)] = e;
+  return quota_fail ? kAssignQuotaFail : kAssignHeapFail;
+}
+
+// The same for one lane alone (entries beyond a workgroup's new-entry list).
+__device__ __forceinline__ int assign_heap_one(const Dev& D, uint32_t e, int ms) {
+  const int64_t quota = D.maps[ms].block_quota;
+  const int held = atomicAdd(&D.mctr[ms].blocks, 1);
+  bool ok = true;
+  bool quota_fail = false;
+  if (quota > 0 && held >= quota) {
+    ok = false;
+    quota_fail = true;
+  }
+  int idx = -1;
+  if (ok) {
+    idx = atomicSub(&D.ctr->heap_top, 1) - 1;
+    if (idx < 0) {
+      atomicAdd(&D.ctr->heap_top, 1);
+      ok = false;
+    }
+  }
+  if (ok) {
+    D.vals[e] = D.heap_free[idx];
+    return kAssignOk;
+  }
+  atomicSub(&D.mctr[ms].blocks, 1);
+  D.vals[e] = -1;
+  D.fail_list[atomicAdd(&D.ctr->fail_n, 1)] = e;
+  return quota_fail ? kAssignQuotaFail : kAssignHeapFail;
 }
 
 // Binning: per point, the truncation segment, the block DDA (every cell is
@@ -216,10 +317,9 @@ __device__ __forceinline__ void note_new(const Dev& D, int e, int map_slot) {
 // one block form one run.  Neighbouring rays share blocks, so each workgroup
 // first gathers its points' blocks in an LDS table: a block is looked up (or
 // inserted) in the map's spatial hash once per workgroup, and reserves the
-// workgroup's records in the block with one 64-bit atomic (runs in the low
-// half, records in the high half).  Each run then learns its record offset
-// in the block, stored with its entry for k_emit.  A full LDS table falls
-// back to per-run global atomics.
+// workgroup's records in the block with one atomic.  Each run then learns
+// its record offset in the block, stored with its entry for k_emit.  A full
+// LDS table falls back to per-run global atomics.
 constexpr int kBinSlots = 1024;
 constexpr uint32_t kPairFail = 0xFFFFFFFFu;
 constexpr uint32_t kPairDirect = 0x80000000u;
@@ -228,6 +328,17 @@ __device__ __forceinline__ unsigned long long lds_block_key(int map_slot, const 
   // 16 bits per axis suffice under the point validity rule (|block| < 2^14)
   return ((unsigned long long)(uint32_t)map_slot << 48) | ((unsigned long long)(kb[0] & 0xFFFF) << 32) |
          ((unsigned long long)(kb[1] & 0xFFFF) << 16) | (unsigned long long)(kb[2] & 0xFFFF);
+}
+
+// Run records of the voxel walk (see k_bin): a run is at most 22 voxels (the
+// longest monotone path through an 8^3 block), so 6 bits of length suffice.
+constexpr int kPackedRuns = 10;
+__device__ __forceinline__ uint32_t run_code(int d0, int d1, int d2) {
+  return (uint32_t)(d0 & 3) | ((uint32_t)(d1 & 3) << 2) | ((uint32_t)(d2 & 3) << 4);
+}
+__device__ __forceinline__ int run_step(uint32_t code, int axis) {
+  const int v = (int)(code >> (2 * axis)) & 3;
+  return v == 3 ? -1 : v;
 }
 
 __device__ __forceinline__ int lds_find_or_add(unsigned long long* tkey, unsigned long long k) {
@@ -248,25 +359,41 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
   __shared__ unsigned long long tkey[kBinSlots];
   __shared__ int tent[kBinSlots];
   __shared__ uint32_t trec[kBinSlots];
-  __shared__ uint32_t trun[kBinSlots];
   __shared__ uint32_t tbase[kBinSlots];
   __shared__ uint32_t lnew[kBinSlots];
   __shared__ int lnew_map[kBinSlots];
   __shared__ uint32_t ltouch[kBinSlots];
   __shared__ uint8_t ltouch_map[kBinSlots];
   __shared__ uint16_t locc[kBinSlots];
-  __shared__ int n_new, n_touch, g_new, g_touch, n_occ;
+  __shared__ int n_new, n_touch, g_touch, n_occ;
   const PassHdr hdr = *D.hdr;
   const int P = hdr.total;
-  unsigned long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int which[8] = {ST_VALID, ST_SKIPPED, ST_BLOCK_CELLS, ST_UPDATES,
-                        ST_TRUNC_VOX, ST_TRUNC_BLK, ST_INSERT_FAIL, ST_UPD_DROPPED};
+  uint32_t st[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // per-thread counts
+  const int which[11] = {ST_VALID, ST_SKIPPED, ST_BLOCK_CELLS, ST_UPDATES, ST_TRUNC_VOX, ST_TRUNC_BLK,
+                         ST_INSERT_FAIL, ST_UPD_DROPPED, ST_NEW_BLOCKS, ST_HEAP_FAIL, ST_QUOTA_FAIL};
+  // heap failures include quota failures, which are also counted apart
+  auto count_assign = [&](int r) {
+    st[8] += r == kAssignOk ? 1 : 0;
+    st[9] += (r == kAssignHeapFail || r == kAssignQuotaFail) ? 1 : 0;
+    st[10] += r == kAssignQuotaFail ? 1 : 0;
+  };
+  // a new entry joins the workgroup's list (heap blocks are handed out for
+  // the whole list at the end of the workgroup's points), or gets its block
+  // at once when the list is full
+  auto note_new = [&](int e, int ms) {
+    const int at = atomicAdd(&n_new, 1);
+    if (at < kBinSlots) {
+      lnew[at] = (uint32_t)e;
+      lnew_map[at] = ms;
+    } else {
+      count_assign(assign_heap_one(D, (uint32_t)e, ms));
+    }
+  };
   const int64_t stride = D.pcap;
   for (int base = blockIdx.x * blockDim.x; base < P; base += gridDim.x * blockDim.x) {
     for (int x = threadIdx.x; x < kBinSlots; x += blockDim.x) {
       tkey[x] = kEmptyKey;
       trec[x] = 0;
-      trun[x] = 0;
     }
     if (threadIdx.x == 0) {
       n_new = 0;
@@ -296,33 +423,31 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
             bool ins;
             const int e = find_or_insert(D, m, pack_key(k0, k1, k2), ins);
             if (e < 0) st[6]++;
-            else if (ins) note_new(D, e, m.slot);
+            else if (ins) note_new(e, m.slot);
           }
           return true;
         });
         st[2] += nb;
         st[5] += tb ? 1 : 0;
-        unsigned long long rkey = kEmptyKey;
-        int32_t rk[3] = {0, 0, 0};
-        int rn = 0;
-        auto close_run = [&]() {
-          if (rn == 0) return;
-          const int slot = lds_find_or_add(tkey, rkey);
+        // A run (the voxels of one block, consecutive along the walk) gets
+        // its record offset in the block's LDS slot, or, past a full table,
+        // straight from the global counter.
+        auto close_run = [&](const int32_t kb[3], int rn) {
+          const int slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb));
           if (slot >= 0) {
             const uint32_t local = atomicAdd(&trec[slot], (uint32_t)rn);
-            atomicAdd(&trun[slot], 1u);
             D.pair_e[(int64_t)nj * stride + i] = (uint32_t)slot;
             D.pair_r[(int64_t)nj * stride + i] = local | ((uint32_t)rn << 24);
           } else {
             bool ins;
-            const int e = find_or_insert(D, m, pack_key(rk[0], rk[1], rk[2]), ins);
+            const int e = find_or_insert(D, m, pack_key(kb[0], kb[1], kb[2]), ins);
             if (e < 0) {
               st[6]++;
               st[7] += rn;
               D.pair_e[(int64_t)nj * stride + i] = kPairFail;
             } else {
-              if (ins) note_new(D, e, m.slot);
-              const unsigned long long old = atomicAdd(&D.blk_cnt[e], ((unsigned long long)rn << 32) | 1ull);
+              if (ins) note_new(e, m.slot);
+              const unsigned long long old = atomicAdd(&D.blk_cnt[e], (unsigned long long)rn << 32);
               if (old == 0) {
                 const int t = atomicAdd(&D.ctr->touched_n, 1);
                 D.touched[t] = (uint32_t)e;
@@ -335,23 +460,60 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
             }
           }
           ++nj;
-          rn = 0;
+        };
+        // The voxel walk only records its runs, in two registers: 6 bits of
+        // length per run, and 6 bits of block step from the previous run (2
+        // bits per axis: consecutive runs' blocks differ by at most one per
+        // axis).  The table work for the runs follows the walk, so the lanes
+        // of a wave do it together rather than at scattered steps of their
+        // walks.  A walk of more than kPackedRuns runs flushes mid-way.
+        unsigned long long lens = 0, steps = 0;
+        int nrun = 0, rn = 0;
+        int32_t fb[3] = {0, 0, 0}, pb[3] = {0, 0, 0};  // flush base block, current run's block
+        auto flush_runs = [&]() {
+          int32_t cb[3] = {fb[0], fb[1], fb[2]};
+          for (int j = 0; j < nrun; ++j) {
+            const uint32_t code = (uint32_t)(steps >> (6 * j)) & 63u;
+            cb[0] += run_step(code, 0);
+            cb[1] += run_step(code, 1);
+            cb[2] += run_step(code, 2);
+            close_run(cb, (int)((lens >> (6 * j)) & 63u));
+          }
+          nrun = 0;
+          lens = 0;
+          steps = 0;
+          fb[0] = cb[0];
+          fb[1] = cb[1];
+          fb[2] = cb[2];
         };
         bool tv = false;
         walk_cells(a, b, m.vs, m.sc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
           const int32_t kb[3] = {block_of(k0), block_of(k1), block_of(k2)};
-          const unsigned long long key = lds_block_key(sd.map, kb);
-          if (key != rkey) {
-            close_run();
-            rkey = key;
-            rk[0] = kb[0];
-            rk[1] = kb[1];
-            rk[2] = kb[2];
+          if (rn == 0) {
+            fb[0] = pb[0] = kb[0];
+            fb[1] = pb[1] = kb[1];
+            fb[2] = pb[2] = kb[2];
+            nrun = 1;
+            rn = 1;
+            return true;
+          }
+          if (kb[0] != pb[0] || kb[1] != pb[1] || kb[2] != pb[2]) {
+            lens |= (unsigned long long)rn << (6 * (nrun - 1));
+            if (nrun == kPackedRuns) flush_runs();
+            steps |= (unsigned long long)run_code(kb[0] - pb[0], kb[1] - pb[1], kb[2] - pb[2]) << (6 * nrun);
+            ++nrun;
+            rn = 0;
+            pb[0] = kb[0];
+            pb[1] = kb[1];
+            pb[2] = kb[2];
           }
           ++rn;
           return true;
         });
-        close_run();
+        if (nrun > 0) {
+          lens |= (unsigned long long)rn << (6 * (nrun - 1));
+          flush_runs();
+        }
         st[4] += tv ? 1 : 0;
       }
       D.pair_n[i] = (uint8_t)nj;
@@ -360,7 +522,7 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
     // ---- phase B: one global lookup and one counter atomic per gathered block ----
     for (int x0 = 0; x0 < kBinSlots; x0 += blockDim.x) {
       const int x = x0 + threadIdx.x;
-      const bool occ = tkey[x] != kEmptyKey;
+      const bool occ = x < kBinSlots && tkey[x] != kEmptyKey;
       const int at = wave_append(&n_occ, occ);
       if (occ) locc[at] = (uint16_t)x;
     }
@@ -379,14 +541,10 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
         st[6]++;
         continue;
       }
-      if (ins) {
-        const int at = atomicAdd(&n_new, 1);
-        lnew[at] = (uint32_t)e;
-        lnew_map[at] = ms;
-      }
+      if (ins) note_new(e, ms);
       const uint32_t nrec = trec[x];
       if (nrec > 0) {
-        const unsigned long long old = atomicAdd(&D.blk_cnt[e], ((unsigned long long)nrec << 32) | trun[x]);
+        const unsigned long long old = atomicAdd(&D.blk_cnt[e], (unsigned long long)nrec << 32);
         tbase[x] = (uint32_t)(old >> 32);
         if (old == 0) {
           const int at = atomicAdd(&n_touch, 1);
@@ -396,15 +554,16 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      g_new = n_new ? atomicAdd(&D.ctr->new_n, n_new) : 0;
-      g_touch = n_touch ? atomicAdd(&D.ctr->touched_n, n_touch) : 0;
+    if (threadIdx.x == 0) g_touch = n_touch ? atomicAdd(&D.ctr->touched_n, n_touch) : 0;
+    {
+      const int nn = min(n_new, kBinSlots);
+      for (int x0 = 0; x0 < nn; x0 += blockDim.x) {
+        const int x = x0 + threadIdx.x;
+        const bool active = x < nn;
+        count_assign(assign_heap(D, active, active ? lnew[x] : 0u, active ? lnew_map[x] : -1));
+      }
     }
     __syncthreads();
-    for (int x = threadIdx.x; x < n_new; x += blockDim.x) {
-      D.new_list[g_new + x] = lnew[x];
-      D.new_map[g_new + x] = lnew_map[x];
-    }
     for (int x = threadIdx.x; x < n_touch; x += blockDim.x) {
       const uint32_t e = ltouch[x];
       D.touched[g_touch + x] = e;
@@ -433,78 +592,6 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
       }
     }
     __syncthreads();
-  }
-  flush_stats(D, which, st);
-}
-
-// Heap block for every entry inserted by the pass (free-list pop; pushes only
-// happen in gc, never concurrently with pops, so the stack needs no ABA
-// guard).  Both counters are updated once per wave: the per-map block count
-// once per distinct map in the wave, the heap stack once; lanes past a quota
-// or past the bottom of the stack hand their share back.
-__global__ void __launch_bounds__(256) k_assign(Dev D) {
-  const int n = ld_acq(&D.ctr->new_n);
-  unsigned long long st[3] = {0, 0, 0};
-  const int which[3] = {ST_NEW_BLOCKS, ST_HEAP_FAIL, ST_QUOTA_FAIL};
-  const int lane = lane_id();
-  const unsigned long long below = (1ull << lane) - 1ull;
-  for (int q0 = blockIdx.x * blockDim.x; q0 < n; q0 += gridDim.x * blockDim.x) {
-    const int q = q0 + threadIdx.x;
-    const bool active = q < n;
-    uint32_t e = 0;
-    int ms = -1;
-    if (active) {
-      e = D.new_list[q];
-      ms = D.new_map[q];
-    }
-    // per-map block count: one atomic per distinct map in the wave
-    int held = 0;  // blocks the map held before this lane's block
-    unsigned long long pending = __ballot(active);
-    while (pending) {
-      const int leader = __ffsll((long long)pending) - 1;
-      const int lm = __shfl(ms, leader);
-      const unsigned long long grp = __ballot(active && ms == lm);
-      int base = 0;
-      if (lane == leader) base = atomicAdd(&D.mctr[lm].blocks, (int)__popcll(grp));
-      base = __shfl(base, leader);
-      if (grp >> lane & 1ull) held = base + (int)__popcll(grp & below);
-      pending &= ~grp;
-    }
-    bool ok = active;
-    if (active) {
-      const int64_t quota = D.maps[ms].block_quota;
-      if (quota > 0 && held >= quota) {
-        atomicSub(&D.mctr[ms].blocks, 1);
-        ok = false;
-        st[2]++;
-      }
-    }
-    // heap pops: one atomic per wave
-    const unsigned long long want = __ballot(ok);
-    const int cnt = (int)__popcll(want);
-    const int leader = want ? __ffsll((long long)want) - 1 : 0;
-    int top = 0;
-    if (want && lane == leader) {
-      top = atomicSub(&D.ctr->heap_top, cnt);
-      const int failed = min(cnt, cnt - max(top, 0));
-      if (failed > 0) atomicAdd(&D.ctr->heap_top, failed);
-    }
-    top = __shfl(top, leader);
-    const int idx = top - cnt + (int)__popcll(want & below);
-    if (ok && idx < 0) {
-      atomicSub(&D.mctr[ms].blocks, 1);
-      ok = false;
-    }
-    if (active) {
-      if (ok) {
-        D.vals[e] = D.heap_free[idx];
-        st[0]++;
-      } else {
-        D.vals[e] = -1;
-        D.keys[e] = kEmptyKey;
-        st[1]++;
-      }
-    }
   }
   flush_stats(D, which, st);
 }
@@ -618,6 +705,13 @@ __device__ __forceinline__ unsigned long long make_rec(int local, int point, flo
   return ((unsigned long long)local << 55) | ((unsigned long long)point << 32) | (unsigned long long)__float_as_uint(d);
 }
 
+// Record offset of run r of point i, or -1 when its block could not be stored.
+constexpr int kEmitRuns = 6;
+__device__ __forceinline__ int64_t run_offset(const Dev& D, int r, int i) {
+  const uint32_t e = D.pair_e[(int64_t)r * D.pcap + i];
+  return e == kPairFail ? -1 : (int64_t)D.seg_off[D.touch_idx[e]] + D.pair_r[(int64_t)r * D.pcap + i];
+}
+
 // Second walk of every point's voxel DDA: each voxel's update record
 // (getDistanceUpdate, H:562-573) is written to its block's record segment at
 // the offset its run reserved in k_bin, so every block's records end up
@@ -635,22 +729,40 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
     const float s[3] = {sd.s[0], sd.s[1], sd.s[2]};
     float a[3], b[3];
     truncation_segment(p, s, m.T, a, b);
+    // absolute record offsets of the point's runs, loaded before the walk
+    // (the first kEmitRuns in registers; later runs, which need a walk far
+    // longer than the truncation band of the workloads, are looked up when
+    // reached)
+    const int nj = D.pair_n[i];
+    int64_t roff[kEmitRuns];
+#pragma unroll
+    for (int r = 0; r < kEmitRuns; ++r) roff[r] = r < nj ? run_offset(D, r, i) : -1;
     unsigned long long rkey = kEmptyKey;
     int j = -1;
     int64_t at = -1;
     bool tv;
+    // cell centres change on one or two axes per step: each axis's centre is
+    // recomputed only when its index moves
+    int32_t kc[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    float vc[3] = {0.f, 0.f, 0.f};
     walk_cells(a, b, m.vs, m.sc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
       const unsigned long long key = pack_key(block_of(k0), block_of(k1), block_of(k2));
       if (key != rkey) {
         rkey = key;
         ++j;
-        const uint32_t e = D.pair_e[(int64_t)j * stride + i];
-        at = e == 0xFFFFFFFFu ? -1 : (int64_t)D.seg_off[D.touch_idx[e]] + D.pair_r[(int64_t)j * stride + i];
+        at = j < kEmitRuns ? -1 : run_offset(D, j, i);
+#pragma unroll
+        for (int r = 0; r < kEmitRuns; ++r)
+          if (r == j) at = roff[r];
       }
       if (at >= 0) {
         const int32_t kk[3] = {k0, k1, k2};
-        const float vc[3] = {cell_centre_of(k0, m.sc_v, m.half_vs), cell_centre_of(k1, m.sc_v, m.half_vs),
-                             cell_centre_of(k2, m.sc_v, m.half_vs)};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+          if (kk[ax] != kc[ax]) {
+            kc[ax] = kk[ax];
+            vc[ax] = cell_centre_of(kk[ax], m.sc_v, m.half_vs);
+          }
         D.seg[at++] = make_rec(local_index(kk), i, signed_distance(p, s, vc));
       }
       return true;
@@ -690,41 +802,76 @@ __device__ __forceinline__ void weight_of(float d, float& w, float& wd) {
 
 __device__ __forceinline__ int rec_voxel(unsigned long long r) { return (int)(r >> 55); }
 
-// Size classes of touched blocks (records per block).
-enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WG, FC_WG2, FC_HUGE, FC_COUNT };
 
-// One descriptor per touched block with its records: everything the fold
-// needs in one 16-byte load.
-__global__ void __launch_bounds__(256) k_fold_prep(Dev D) {
+// Record segments and fold descriptors of the touched blocks.  Per tile of
+// the touched list (tile totals from k_scan_tiles): the tile's carry is the
+// sum of the preceding totals (summed by the workgroup itself, so no
+// separate top-level scan is launched), each block's segment offset follows
+// by a workgroup scan, and its 16-byte descriptor {record base, records,
+// heap block, map} is filed under its size class with one counter atomic
+// per class and workgroup.  The pass counters of the blocks are reset for
+// the next pass, and entries that got no heap block leave the table.
+__global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
+  __shared__ uint32_t sh[16];
   __shared__ int lcnt[FC_COUNT], gbase[FC_COUNT];
-  const int nt = ld_acq(&D.ctr->touched_n);
+  const int n = ld_acq(&D.ctr->touched_n);
+  const int ntiles = (n + kScanTile - 1) / kScanTile;
   unsigned long long dropped = 0;
-  for (int b0 = blockIdx.x * blockDim.x; b0 < nt; b0 += gridDim.x * blockDim.x) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    uint32_t c = 0;
+    for (int k = threadIdx.x; k < t; k += blockDim.x) c += D.tile_sums[k];
+    uint32_t carry;
+    block_scan_excl(c, sh, carry);
     if (threadIdx.x < FC_COUNT) lcnt[threadIdx.x] = 0;
-    __syncthreads();
-    const int t = b0 + threadIdx.x;
-    int cls = -1, rank = 0;
-    int4 d = make_int4(0, 0, 0, 0);
-    if (t < nt) {
-      const uint32_t e = D.touched[t];
-      const uint32_t base = D.seg_off[t];
-      const int n = (int)(D.seg_off[t + 1] - base);
-      const int slot = D.vals[e];
-      D.blk_cnt[e] = 0;
-      if (slot < 0) {
-        dropped += n;
-      } else {
-        d = make_int4((int)base, n, slot, (int)D.touched_map[t]);
-        cls = n <= 16 ? FC_16 : n <= 32 ? FC_32 : n <= 64 ? FC_64 : n <= kCountTileS ? FC_WG : n <= kCountTile ? FC_WG2 : FC_HUGE;
-        rank = atomicAdd(&lcnt[cls], 1);
+    const int i0 = t * kScanTile + threadIdx.x * 4;
+    uint32_t e4[4], cn[4];
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      e4[k] = 0;
+      cn[k] = 0;
+      if (i0 + k < n) {
+        e4[k] = D.touched[i0 + k];
+        cn[k] = (uint32_t)(D.blk_cnt[e4[k]] >> 32);
       }
+      v += cn[k];
     }
+    uint32_t tot;
+    uint32_t ex = block_scan_excl(v, sh, tot) + carry;
+    int cls[4], rank[4];
+    int4 dsc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      cls[k] = -1;
+      rank[k] = 0;
+      dsc[k] = make_int4(0, 0, 0, 0);
+      if (i0 + k < n) {
+        D.seg_off[i0 + k] = ex;
+        const int slot = D.vals[e4[k]];
+        D.blk_cnt[e4[k]] = 0;
+        if (slot < 0) {
+          dropped += cn[k];
+        } else {
+          dsc[k] = make_int4((int)ex, (int)cn[k], slot, (int)D.touched_map[i0 + k]);
+          cls[k] = fold_class((int)cn[k]);
+          rank[k] = atomicAdd(&lcnt[cls[k]], 1);
+        }
+      }
+      ex += cn[k];
+    }
+    if (t == ntiles - 1 && threadIdx.x == 0) D.seg_off[n] = carry + tot;
     __syncthreads();
-    if (threadIdx.x < FC_COUNT) gbase[threadIdx.x] = lcnt[threadIdx.x] ? atomicAdd(&D.ctr->fold_n[threadIdx.x], lcnt[threadIdx.x]) : 0;
+    if (threadIdx.x < FC_COUNT)
+      gbase[threadIdx.x] = lcnt[threadIdx.x] ? atomicAdd(&D.ctr->fold_n[threadIdx.x], lcnt[threadIdx.x]) : 0;
     __syncthreads();
-    if (cls >= 0) D.fold_desc[(int64_t)cls * D.fold_cap + gbase[cls] + rank] = d;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (cls[k] >= 0) D.fold_desc[(int64_t)cls[k] * D.fold_cap + gbase[cls[k]] + rank[k]] = dsc[k];
     __syncthreads();
   }
+  const int nf = ld_acq(&D.ctr->fail_n);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nf; q += gridDim.x * blockDim.x)
+    D.keys[D.fail_list[q]] = kEmptyKey;
   stat_add(D, ST_UPD_DROPPED, dropped);
 }
 
@@ -733,16 +880,16 @@ __global__ void __launch_bounds__(256) k_fold_prep(Dev D) {
 // run of one voxel is folded by its first lane pulling the run's weight
 // terms from the following lanes.
 template <class State, int G>
-__global__ void __launch_bounds__(256) k_fold_small(Dev D) {
+__device__ __forceinline__ void fold_small_class(const Dev& D, unsigned long long& uniq) {
   constexpr int kPer = 64 / G;
+  constexpr int kCls = G == 16 ? FC_16 : G == 32 ? FC_32 : FC_64;
   const int lane = lane_id();
   const int gl = lane & (G - 1);
   const int grp = lane / G;
-  const int n_desc = ld_acq(&D.ctr->fold_n[G == 16 ? FC_16 : G == 32 ? FC_32 : FC_64]);
-  const int4* desc = D.fold_desc + (int64_t)(G == 16 ? FC_16 : G == 32 ? FC_32 : FC_64) * D.fold_cap;
+  const int n_desc = ld_acq(&D.ctr->fold_n[kCls]);
+  const int4* desc = D.fold_desc + (int64_t)kCls * D.fold_cap;
   const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int nw = gridDim.x * (blockDim.x >> 6);
-  unsigned long long uniq = 0;
   for (int q0 = gw * kPer; q0 < n_desc; q0 += nw * kPer) {
     const int q = q0 + grp;
     int4 d = make_int4(0, 0, 0, 0);
@@ -785,7 +932,6 @@ __global__ void __launch_bounds__(256) k_fold_small(Dev D) {
       uniq++;
     }
   }
-  stat_add(D, ST_UNIQUE, uniq);
 }
 
 __device__ __forceinline__ void bitonic_sort_lds(unsigned long long* a, int n) {
@@ -806,160 +952,189 @@ __device__ __forceinline__ void bitonic_sort_lds(unsigned long long* a, int n) {
   }
 }
 
-// Blocks of 65..kCountTile records, one workgroup each: a counting sort by
-// voxel in LDS (histogram, scan, scatter) groups every voxel's updates;
-// each voxel's thread then orders its few updates by point (insertion sort:
-// a voxel receives at most a few dozen updates per pass) and folds them.
-template <class State, int TILE, int CLS>
-__global__ void __launch_bounds__(256) k_fold_count(Dev D) {
-  __shared__ unsigned long long rin[TILE];
-  __shared__ unsigned long long rout[TILE];
+// One block of 65..TILE records, one workgroup: a counting sort by voxel in
+// LDS (histogram, scan, scatter) groups every voxel's updates; each record
+// is then placed by its rank (by point) inside its voxel, and each voxel's
+// thread folds its updates in point order.  rin/rout hold TILE records.
+template <class State, int TILE>
+__device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, unsigned long long* rin,
+                                                 unsigned long long* rout, uint32_t* cnt, uint32_t* start,
+                                                 uint32_t* sh, unsigned long long& uniq) {
+  const int c = d.y;
+  const unsigned long long* sg = D.seg + (uint32_t)d.x;
+  for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) cnt[l] = 0;
+  __syncthreads();
+  for (int x = threadIdx.x; x < c; x += blockDim.x) {
+    const unsigned long long r = sg[x];
+    rin[x] = r;
+    atomicAdd(&cnt[rec_voxel(r)], 1u);
+  }
+  __syncthreads();
+  {
+    const int l0 = 2 * threadIdx.x;
+    const uint32_t c0 = cnt[l0], c1 = cnt[l0 + 1];
+    uint32_t tot;
+    const uint32_t ex = block_scan_excl(c0 + c1, sh, tot);
+    start[l0] = ex;
+    start[l0 + 1] = ex + c0;
+    cnt[l0] = 0;
+    cnt[l0 + 1] = 0;
+    if (threadIdx.x == 0) start[kVoxelsPerBlock] = tot;
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < c; x += blockDim.x) {
+    const unsigned long long r = rin[x];
+    const int l = rec_voxel(r);
+    rout[start[l] + atomicAdd(&cnt[l], 1u)] = r;
+  }
+  __syncthreads();
+  // point order inside each voxel: a record's place is the number of records
+  // of its voxel that sort before it (records of one voxel are distinct, a
+  // walk visits a cell once)
+  for (int x = threadIdx.x; x < c; x += blockDim.x) {
+    const unsigned long long r = rout[x];
+    const int l = rec_voxel(r);
+    const int b0 = (int)start[l], b1 = (int)start[l + 1];
+    int rank = 0;
+    for (int j = b0; j < b1; ++j) rank += rout[j] < r;
+    rin[b0 + rank] = r;
+  }
+  __syncthreads();
+  const MapDev& m = D.maps[d.w];
+  const float maxW = m.maxW, Tt = m.T, invT = 1.0f / m.T;
+  for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) {
+    const int b0 = (int)start[l], b1 = (int)start[l + 1];
+    if (b1 == b0) continue;
+    const int64_t vi = (int64_t)d.z * kVoxelsPerBlock + l;
+    float S, W;
+    State::load(D.heap, vi, invT, Tt, S, W);
+    for (int i = b0; i < b1; ++i) {
+      float w, wd;
+      weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
+      fold_step(S, W, w, wd, maxW);
+    }
+    State::store(D.heap, vi, invT, Tt, S, W);
+    uniq++;
+  }
+  __syncthreads();
+}
+
+// One block beyond the counting tile: records sorted in LDS tiles of
+// kFoldCap written back in place, then every thread merges its voxel's runs
+// across the tiles in point order.  sk holds kFoldCap records.
+template <class State>
+__device__ __forceinline__ void fold_huge_block(const Dev& D, const int4 d, unsigned long long* sk,
+                                                unsigned long long& uniq, unsigned long long& dropped) {
+  const int c = d.y;
+  const int slot = d.z;
+  const int nch = (c + kFoldCap - 1) / kFoldCap;
+  __syncthreads();
+  if (nch > kMaxChunks) {
+    if (threadIdx.x == 0) dropped += c;
+    return;
+  }
+  unsigned long long* sg = D.seg + (uint32_t)d.x;
+  const MapDev& m = D.maps[d.w];
+  const float maxW = m.maxW, Tt = m.T, invT = 1.0f / m.T;
+  for (int ch = 0; ch < nch; ++ch) {
+    const int len = min(kFoldCap, c - ch * kFoldCap);
+    const int n2 = next_pow2(len);
+    for (int x = threadIdx.x; x < n2; x += blockDim.x) sk[x] = x < len ? sg[ch * kFoldCap + x] : ~0ull;
+    __syncthreads();
+    bitonic_sort_lds(sk, n2);
+    for (int x = threadIdx.x; x < len; x += blockDim.x) sg[ch * kFoldCap + x] = sk[x];
+    __threadfence();
+    __syncthreads();
+  }
+  for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) {
+    int cur[kMaxChunks], end[kMaxChunks];
+    bool any = false;
+    for (int ch = 0; ch < nch; ++ch) {
+      const int lo0 = ch * kFoldCap, hi0 = min(c, lo0 + kFoldCap);
+      int lo = lo0, hi = hi0;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (rec_voxel(ld_acq(&sg[mid])) < l) lo = mid + 1;
+        else hi = mid;
+      }
+      int lo2 = lo;
+      hi = hi0;
+      while (lo2 < hi) {
+        const int mid = (lo2 + hi) >> 1;
+        if (rec_voxel(ld_acq(&sg[mid])) < l + 1) lo2 = mid + 1;
+        else hi = mid;
+      }
+      cur[ch] = lo;
+      end[ch] = lo2;
+      any = any || (lo2 > lo);
+    }
+    if (!any) continue;
+    const int64_t vi = (int64_t)slot * kVoxelsPerBlock + l;
+    float S, W;
+    State::load(D.heap, vi, invT, Tt, S, W);
+    for (;;) {
+      int best = -1;
+      unsigned long long bv = ~0ull;
+      for (int ch = 0; ch < nch; ++ch) {
+        if (cur[ch] < end[ch]) {
+          const unsigned long long v = ld_acq(&sg[cur[ch]]);
+          if (best < 0 || v < bv) {
+            best = ch;
+            bv = v;
+          }
+        }
+      }
+      if (best < 0) break;
+      float w, wd;
+      weight_of(__uint_as_float((uint32_t)bv), w, wd);
+      fold_step(S, W, w, wd, maxW);
+      cur[best]++;
+    }
+    State::store(D.heap, vi, invT, Tt, S, W);
+    uniq++;
+  }
+  __syncthreads();
+}
+
+// Fold, first launch: blocks of 65..kCountTileS records (one workgroup
+// each, 20 KB of LDS, so up to eight workgroups share a CU), then the small
+// classes (several blocks per wave).
+template <class State>
+__global__ void __launch_bounds__(256) k_fold_a(Dev D) {
+  __shared__ unsigned long long rin[kCountTileS];
+  __shared__ unsigned long long rout[kCountTileS];
   __shared__ uint32_t cnt[kVoxelsPerBlock];
   __shared__ uint32_t start[kVoxelsPerBlock + 1];
   __shared__ uint32_t sh[16];
-  const int n_desc = ld_acq(&D.ctr->fold_n[CLS]);
-  const int4* desc = D.fold_desc + (int64_t)CLS * D.fold_cap;
   unsigned long long uniq = 0;
-  for (int q = blockIdx.x; q < n_desc; q += gridDim.x) {
-    const int4 d = desc[q];
-    const int c = d.y;
-    const unsigned long long* sg = D.seg + (uint32_t)d.x;
-    for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) cnt[l] = 0;
-    __syncthreads();
-    for (int x = threadIdx.x; x < c; x += blockDim.x) {
-      const unsigned long long r = sg[x];
-      rin[x] = r;
-      atomicAdd(&cnt[rec_voxel(r)], 1u);
-    }
-    __syncthreads();
-    {
-      const int l0 = 2 * threadIdx.x;
-      const uint32_t c0 = cnt[l0], c1 = cnt[l0 + 1];
-      uint32_t tot;
-      const uint32_t ex = block_scan_excl(c0 + c1, sh, tot);
-      start[l0] = ex;
-      start[l0 + 1] = ex + c0;
-      cnt[l0] = 0;
-      cnt[l0 + 1] = 0;
-      if (threadIdx.x == 0) start[kVoxelsPerBlock] = tot;
-    }
-    __syncthreads();
-    for (int x = threadIdx.x; x < c; x += blockDim.x) {
-      const unsigned long long r = rin[x];
-      const int l = rec_voxel(r);
-      rout[start[l] + atomicAdd(&cnt[l], 1u)] = r;
-    }
-    __syncthreads();
-    const MapDev& m = D.maps[d.w];
-    const float maxW = m.maxW, Tt = m.T, invT = 1.0f / m.T;
-    for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) {
-      const int b0 = (int)start[l], b1 = (int)start[l + 1];
-      if (b1 == b0) continue;
-      for (int i = b0 + 1; i < b1; ++i) {
-        const unsigned long long key = rout[i];
-        int j = i - 1;
-        while (j >= b0 && rout[j] > key) {
-          rout[j + 1] = rout[j];
-          --j;
-        }
-        rout[j + 1] = key;
-      }
-      const int64_t vi = (int64_t)d.z * kVoxelsPerBlock + l;
-      float S, W;
-      State::load(D.heap, vi, invT, Tt, S, W);
-      for (int i = b0; i < b1; ++i) {
-        float w, wd;
-        weight_of(__uint_as_float((uint32_t)rout[i]), w, wd);
-        fold_step(S, W, w, wd, maxW);
-      }
-      State::store(D.heap, vi, invT, Tt, S, W);
-      uniq++;
-    }
-    __syncthreads();
-  }
+  const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG]);
+  const int4* desc = D.fold_desc + (int64_t)FC_WG * D.fold_cap;
+  for (int q = blockIdx.x; q < n_wg; q += gridDim.x)
+    fold_count_block<State, kCountTileS>(D, desc[q], rin, rout, cnt, start, sh, uniq);
+  fold_small_class<State, 64>(D, uniq);
+  fold_small_class<State, 32>(D, uniq);
+  fold_small_class<State, 16>(D, uniq);
   stat_add(D, ST_UNIQUE, uniq);
 }
 
-// Blocks beyond the counting tile: records sorted in LDS tiles written back
-// in place, then every thread merges its voxel's runs across the tiles in point
-// order.
+// Fold, second launch: blocks of kCountTileS+1..kCountTile records, then the
+// larger ones; one 64 KB LDS buffer serves as the counting tile pair or the
+// sort tile.
 template <class State>
-__global__ void __launch_bounds__(kFoldThreads) k_fold_huge(Dev D) {
-  __shared__ unsigned long long sk[kFoldCap];
-  const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
-  const int4* desc = D.fold_desc + (int64_t)FC_HUGE * D.fold_cap;
+__global__ void __launch_bounds__(256) k_fold_b(Dev D) {
+  static_assert(2 * kCountTile <= kFoldCap, "counting tiles share the sort buffer");
+  __shared__ unsigned long long buf[kFoldCap];
+  __shared__ uint32_t cnt[kVoxelsPerBlock];
+  __shared__ uint32_t start[kVoxelsPerBlock + 1];
+  __shared__ uint32_t sh[16];
   unsigned long long uniq = 0, dropped = 0;
-  for (int qb = blockIdx.x; qb < n_huge; qb += gridDim.x) {
-    const int4 d = desc[qb];
-    const int c = d.y;
-    const int slot = d.z;
-    const int nch = (c + kFoldCap - 1) / kFoldCap;
-    __syncthreads();
-    if (nch > kMaxChunks) {
-      if (threadIdx.x == 0) dropped += c;
-      continue;
-    }
-    unsigned long long* sg = D.seg + (uint32_t)d.x;
-    const MapDev& m = D.maps[d.w];
-    const float maxW = m.maxW, Tt = m.T, invT = 1.0f / m.T;
-    for (int ch = 0; ch < nch; ++ch) {
-      const int len = min(kFoldCap, c - ch * kFoldCap);
-      const int n2 = next_pow2(len);
-      for (int x = threadIdx.x; x < n2; x += blockDim.x) sk[x] = x < len ? sg[ch * kFoldCap + x] : ~0ull;
-      __syncthreads();
-      bitonic_sort_lds(sk, n2);
-      for (int x = threadIdx.x; x < len; x += blockDim.x) sg[ch * kFoldCap + x] = sk[x];
-      __threadfence();
-      __syncthreads();
-    }
-    for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) {
-      int cur[kMaxChunks], end[kMaxChunks];
-      bool any = false;
-      for (int ch = 0; ch < nch; ++ch) {
-        const int lo0 = ch * kFoldCap, hi0 = min(c, lo0 + kFoldCap);
-        int lo = lo0, hi = hi0;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (rec_voxel(ld_acq(&sg[mid])) < l) lo = mid + 1;
-          else hi = mid;
-        }
-        int lo2 = lo;
-        hi = hi0;
-        while (lo2 < hi) {
-          const int mid = (lo2 + hi) >> 1;
-          if (rec_voxel(ld_acq(&sg[mid])) < l + 1) lo2 = mid + 1;
-          else hi = mid;
-        }
-        cur[ch] = lo;
-        end[ch] = lo2;
-        any = any || (lo2 > lo);
-      }
-      if (!any) continue;
-      const int64_t vi = (int64_t)slot * kVoxelsPerBlock + l;
-      float S, W;
-      State::load(D.heap, vi, invT, Tt, S, W);
-      for (;;) {
-        int best = -1;
-        unsigned long long bv = ~0ull;
-        for (int ch = 0; ch < nch; ++ch) {
-          if (cur[ch] < end[ch]) {
-            const unsigned long long v = ld_acq(&sg[cur[ch]]);
-            if (best < 0 || v < bv) {
-              best = ch;
-              bv = v;
-            }
-          }
-        }
-        if (best < 0) break;
-        float w, wd;
-        weight_of(__uint_as_float((uint32_t)bv), w, wd);
-        fold_step(S, W, w, wd, maxW);
-        cur[best]++;
-      }
-      State::store(D.heap, vi, invT, Tt, S, W);
-      uniq++;
-    }
-  }
+  const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG2]);
+  const int4* desc = D.fold_desc + (int64_t)FC_WG2 * D.fold_cap;
+  for (int q = blockIdx.x; q < n_wg; q += gridDim.x)
+    fold_count_block<State, kCountTile>(D, desc[q], buf, buf + kCountTile, cnt, start, sh, uniq);
+  const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
+  const int4* hdesc = D.fold_desc + (int64_t)FC_HUGE * D.fold_cap;
+  for (int q = blockIdx.x; q < n_huge; q += gridDim.x) fold_huge_block<State>(D, hdesc[q], buf, uniq, dropped);
   stat_add(D, ST_UNIQUE, uniq);
   stat_add(D, ST_UPD_DROPPED, dropped);
 }

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -49,6 +50,8 @@ constexpr int kMaxMaps = 256;
 constexpr int kRing = 16;          // staged descriptor slots
 constexpr int kMaxStride = 128;    // largest MAX_VOXELS_TRAVERSED a map may have
 constexpr int kTimingRing = 256;
+// statistics an integration pass produces (publish and gc keep their own)
+constexpr uint32_t kPassStats = ((1u << ST_COUNT) - 1u) & ~((1u << ST_PUBLISHED) | (1u << ST_GC_REMOVED));
 
 struct DescSlot {
   PassHdr hdr;
@@ -110,8 +113,7 @@ struct tsdf_engine {
   int32_t* freed = nullptr;
 
   // pass scratch
-  PassHdr* hdr_dev = nullptr;
-  ScanDev* scans_dev = nullptr;
+  DescSlot* desc_dev = nullptr;  // the pass header and scan descriptors, one copy per pass
   DescSlot* stage = nullptr;  // pinned [kRing]
   hipEvent_t stage_ev[kRing] = {};
   bool stage_used[kRing] = {};
@@ -126,9 +128,7 @@ struct tsdf_engine {
   uint32_t* touched = nullptr;
   uint32_t* seg_off = nullptr;
   uint32_t* tile_sums = nullptr;
-  uint32_t* new_list = nullptr;
-  int32_t* new_map = nullptr;
-  uint32_t* big_list = nullptr;
+  uint32_t* fail_list = nullptr;
   int4* fold_desc = nullptr;
   unsigned long long* part = nullptr;
   int64_t fold_cap = 0;
@@ -180,8 +180,8 @@ struct tsdf_engine {
     D.heap = heap;
     D.heap_free = heap_free;
     D.freed = freed;
-    D.hdr = hdr_dev;
-    D.scans = scans_dev;
+    D.hdr = &desc_dev->hdr;
+    D.scans = desc_dev->scans;
     D.pair_e = pair_e;
     D.pair_r = pair_r;
     D.pair_n = pair_n;
@@ -189,9 +189,7 @@ struct tsdf_engine {
     D.touched = touched;
     D.seg_off = seg_off;
     D.tile_sums = tile_sums;
-    D.new_list = new_list;
-    D.new_map = new_map;
-    D.big_list = big_list;
+    D.fail_list = fail_list;
     D.fold_desc = fold_desc;
     D.part = part;
     D.fold_cap = fold_cap;
@@ -350,8 +348,8 @@ int stage_descs(tsdf_engine* E, const PassHdr& hdr, const ScanDev* scans, int ns
   DescSlot& S = E->stage[r];
   S.hdr = hdr;
   std::memcpy(S.scans, scans, sizeof(ScanDev) * nscans);
-  HIPCHK(hipMemcpyAsync(E->hdr_dev, &S.hdr, sizeof(PassHdr), hipMemcpyHostToDevice, E->stream));
-  HIPCHK(hipMemcpyAsync(E->scans_dev, S.scans, sizeof(ScanDev) * std::max(nscans, 1), hipMemcpyHostToDevice, E->stream));
+  const size_t bytes = offsetof(DescSlot, scans) + sizeof(ScanDev) * std::max(nscans, 1);
+  HIPCHK(hipMemcpyAsync(E->desc_dev, &S, bytes, hipMemcpyHostToDevice, E->stream));
   HIPCHK(hipEventRecord(E->stage_ev[r], E->stream));
   E->stage_used[r] = true;
   return TSDF_OK;
@@ -363,43 +361,29 @@ int enqueue_pass(tsdf_engine* E, int64_t P, int tid) {
   Dev D = E->view();
   hipStream_t st = E->stream;
   const tsdf_launch_config& L = E->launch;
-  const uint32_t pass_stats = ((1u << ST_COUNT) - 1u) & ~((1u << ST_PUBLISHED) | (1u << ST_GC_REMOVED));
-  hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(64), 0, st, D);
-  hipLaunchKernelGGL(k_stats_zero, dim3(64), dim3(256), 0, st, D, pass_stats);
+  hipLaunchKernelGGL(k_pass_begin, dim3(64), dim3(256), 0, st, D, kPassStats);
   CHK(timing_mark(E, tid, 0));
   const int tb = L.traverse_block;
   hipLaunchKernelGGL(k_bin, dim3(grid_for(P, tb, L.traverse_grid_cap)), dim3(tb), 0, st, D);
   CHK(timing_mark(E, tid, 1));
-  hipLaunchKernelGGL(k_assign, dim3(grid_for(P, 256, 1024)), dim3(256), 0, st, D);
+  // (heap assignment runs inside k_bin: the assign stage is empty)
   CHK(timing_mark(E, tid, 2));
   const int64_t tmax = std::min<int64_t>(E->entries, P * E->stride_cap);
   const int sg = grid_for(tmax, kScanTile, 1024);
   CountTouchedRecords ct{E->touched, E->blk_cnt};
   hipLaunchKernelGGL(k_scan_tiles<CountTouchedRecords>, dim3(sg), dim3(256), 0, st, &E->ctr->touched_n, ct, E->tile_sums);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, &E->ctr->touched_n, E->tile_sums);
-  hipLaunchKernelGGL(k_scan_apply<CountTouchedRecords>, dim3(sg), dim3(256), 0, st, &E->ctr->touched_n, ct, E->tile_sums,
-                     E->seg_off);
+  hipLaunchKernelGGL(k_seg_apply, dim3(sg), dim3(256), 0, st, D);
   CHK(timing_mark(E, tid, 3));
   hipLaunchKernelGGL(k_emit, dim3(grid_for(P, 256, 65536)), dim3(256), 0, st, D);
   CHK(timing_mark(E, tid, 4));
   const int fg = std::max(1, L.fold_grid);
-  hipLaunchKernelGGL(k_fold_prep, dim3(grid_for(tmax, 256, 2048)), dim3(256), 0, st, D);
   if (E->state_format == TSDF_STATE_F16) {
-    hipLaunchKernelGGL((k_fold_small<StateF16, 16>), dim3(fg), dim3(256), 0, st, D);
-    hipLaunchKernelGGL((k_fold_small<StateF16, 32>), dim3(fg), dim3(256), 0, st, D);
-    hipLaunchKernelGGL((k_fold_small<StateF16, 64>), dim3(fg), dim3(256), 0, st, D);
-    hipLaunchKernelGGL((k_fold_count<StateF16, kCountTileS, FC_WG>), dim3(2048), dim3(256), 0, st, D);
-    hipLaunchKernelGGL((k_fold_count<StateF16, kCountTile, FC_WG2>), dim3(512), dim3(256), 0, st, D);
-    hipLaunchKernelGGL(k_fold_huge<StateF16>, dim3(64), dim3(kFoldThreads), 0, st, D);
+    hipLaunchKernelGGL(k_fold_a<StateF16>, dim3(fg), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_fold_b<StateF16>, dim3(512), dim3(256), 0, st, D);
   } else {
-    hipLaunchKernelGGL((k_fold_small<StateF32, 16>), dim3(fg), dim3(256), 0, st, D);
-    hipLaunchKernelGGL((k_fold_small<StateF32, 32>), dim3(fg), dim3(256), 0, st, D);
-    hipLaunchKernelGGL((k_fold_small<StateF32, 64>), dim3(fg), dim3(256), 0, st, D);
-    hipLaunchKernelGGL((k_fold_count<StateF32, kCountTileS, FC_WG>), dim3(2048), dim3(256), 0, st, D);
-    hipLaunchKernelGGL((k_fold_count<StateF32, kCountTile, FC_WG2>), dim3(512), dim3(256), 0, st, D);
-    hipLaunchKernelGGL(k_fold_huge<StateF32>, dim3(64), dim3(kFoldThreads), 0, st, D);
+    hipLaunchKernelGGL(k_fold_a<StateF32>, dim3(fg), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_fold_b<StateF32>, dim3(512), dim3(256), 0, st, D);
   }
-  hipLaunchKernelGGL(k_stats_reduce, dim3(ST_COUNT), dim3(256), 0, st, D, pass_stats);
   CHK(timing_mark(E, tid, 5));
   HIPCHK(hipGetLastError());
   return TSDF_OK;
@@ -709,10 +693,10 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
       (r = dev_alloc(&E->touch_idx, entries)) || (r = dev_alloc(&E->next, c.total_buckets)) ||
       (r = dev_alloc(&E->ovf_stack, c.total_buckets)) || (r = dev_alloc(&E->orphans, c.total_buckets)) ||
       (r = dev_alloc(&E->heap_free, c.heap_blocks)) || (r = dev_alloc(&E->freed, c.heap_blocks)) ||
-      (r = dev_alloc(&E->hdr_dev, 1)) || (r = dev_alloc(&E->scans_dev, kMaxScansPerBatch)) ||
+      (r = dev_alloc(&E->desc_dev, 1)) ||
       (r = dev_alloc(&E->touched, entries)) || (r = dev_alloc(&E->seg_off, entries + 1)) ||
-      (r = dev_alloc(&E->tile_sums, entries / kScanTile + 2)) || (r = dev_alloc(&E->new_list, entries)) ||
-      (r = dev_alloc(&E->new_map, entries)) || (r = dev_alloc(&E->big_list, entries)) || (r = dev_alloc(&E->touched_map, entries)) || (r = dev_alloc(&E->part, ST_COUNT * kPartSlots)) ||
+      (r = dev_alloc(&E->tile_sums, entries / kScanTile + 2)) || (r = dev_alloc(&E->fail_list, entries)) ||
+      (r = dev_alloc(&E->touched_map, entries)) || (r = dev_alloc(&E->part, ST_COUNT * kPartSlots)) ||
       (r = dev_alloc(&E->sel, entries)) || (r = dev_alloc(&E->pub_cnt, entries + 1)) ||
       (r = dev_alloc(&E->pub_off, entries + 1)))
     return cleanup(r);
@@ -740,6 +724,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
       hipMemsetAsync(E->mctr, 0, sizeof(MapCtr) * kMaxMaps, st) != hipSuccess ||
       hipMemsetAsync(E->heap, 0, (size_t)vbytes, st) != hipSuccess ||
       hipMemsetAsync(E->blk_cnt, 0, sizeof(unsigned long long) * entries, st) != hipSuccess ||
+      hipMemsetAsync(E->part, 0, sizeof(unsigned long long) * ST_COUNT * kPartSlots, st) != hipSuccess ||
       hipMemsetAsync(E->keys, 0xFF, sizeof(unsigned long long) * entries, st) != hipSuccess ||
       hipMemsetAsync(E->vals, 0xFF, sizeof(int32_t) * entries, st) != hipSuccess ||
       hipMemsetAsync(E->next, 0xFF, sizeof(int32_t) * c.total_buckets, st) != hipSuccess)
@@ -767,8 +752,8 @@ int tsdf_engine_destroy(tsdf_engine* E) {
       E->map_by_slot[i] = nullptr;
     }
   void* ptrs[] = {E->maps_dev, E->mctr, E->ctr, E->keys, E->vals, E->blk_cnt, E->touch_idx, E->next, E->ovf_stack,
-                  E->orphans, E->heap, E->heap_free, E->freed, E->hdr_dev, E->scans_dev, E->pair_e, E->pair_r, E->pair_n,
-                  E->seg, E->touched, E->seg_off, E->tile_sums, E->new_list, E->new_map, E->big_list, E->fold_desc, E->part, E->touched_map, E->sel,
+                  E->orphans, E->heap, E->heap_free, E->freed, E->desc_dev, E->pair_e, E->pair_r, E->pair_n,
+                  E->seg, E->touched, E->seg_off, E->tile_sums, E->fail_list, E->fold_desc, E->part, E->touched_map, E->sel,
                   E->pub_cnt, E->pub_off, E->pub_buf};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -944,6 +929,10 @@ int tsdf_process_scan(tsdf_map* m, const float* points_host, int64_t n, const fl
 int tsdf_last_stats(tsdf_engine* E, tsdf_scan_stats* out) {
   if (!E || !out) return fail(TSDF_ERR_INVALID_ARG, "null argument");
   CHK(set_device(E));
+  // the last pass's statistics stay in their partial slots until the next
+  // pass starts: reduce them now
+  hipLaunchKernelGGL(k_stats_reduce, dim3(ST_COUNT), dim3(256), 0, E->stream, E->view(), kPassStats);
+  HIPCHK(hipGetLastError());
   CHK(read_ctr(E));
   const unsigned long long* s = E->ctr_host->stats;
   out->points = E->last_points;
