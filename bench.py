@@ -73,7 +73,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--only", default="all", choices=["all", "plain"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--error-scans", type=int, default=40, help="dense scans for the FP16/FP32 error figures")
+    ap.add_argument("--error-scans", type=int, default=200,
+                    help="dense scans integrated before the FP16/FP32 error figures (also the CPU-baseline sample)")
     ap.add_argument("--serve-requests", type=int, default=2000)
     args = ap.parse_args()
 
@@ -96,7 +97,7 @@ def main():
         smaps_pts, smaps_sens = workloads.generate_maps(configs.DENSE, configs.SERVE_MAPS, 24)
     log("generated workloads in %.1fs (dense miss-fill fraction %.2f)" % (time.perf_counter() - t_gen, float(np.mean(fills))))
 
-    from ros_tsdf_amd import BatchGraph, Engine, abi, serve
+    from ros_tsdf_amd import BatchGraph, Engine, abi, autotune, serve
 
     result = {}
     # ---------------- plain per-scan, FP32 state ----------------
@@ -190,12 +191,11 @@ def main():
             batches.append(sc)
             scans_all.extend(sc)
         bpool = make_pool(scans_all)
-        eng = Engine(heap_blocks=1 << 21, total_buckets=1 << 21, max_batch_points=1 << 22)
-        maps = [eng.map(configs.DENSE_PARAMS, buckets_log2=16) for _ in range(configs.BATCH_MAPS)]
+        # hash pool large enough for the tuner's largest tables (8 maps x 2^21 home buckets + overflow)
+        eng = Engine(heap_blocks=1 << 21, total_buckets=1 << 25, max_batch_points=1 << 22)
         bbuf = eng.buffer(bpool.points.nbytes)
         bbuf.upload(bpool.points)
         max_pts = max(sum(s[0].shape[0] for s in b) for b in batches)
-        g = BatchGraph(eng, maps, max_points=max_pts, max_scans=configs.BATCH_SCANS)
         descs = []
         j = 0
         for b in batches:
@@ -204,14 +204,16 @@ def main():
                 d.append((mi, bpool.counts[j], bpool.offsets[j], bpool.sensors[j]))
                 j += 1
             descs.append(d)
-        # warm-up on the first batch, then all batches timed
-        g.launch(descs[0], bbuf)
-        eng.synchronize()
-        for mm in maps:
-            mm.close()
-        maps = [eng.map(configs.DENSE_PARAMS, buckets_log2=16) for _ in range(configs.BATCH_MAPS)]
-        g.close()
+        # autotuner: launch shapes and table shapes, measured on the first four batches
+        t_tune = time.perf_counter()
+        best, hist = autotune.tune(eng, configs.DENSE_PARAMS, configs.BATCH_MAPS, descs[:4], bbuf, reps=3, log=log)
+        t_tune = time.perf_counter() - t_tune
+        log("autotune picked %s in %.1f s" % (best, t_tune))
+        mk = lambda: [eng.map(configs.DENSE_PARAMS, buckets_log2=best["buckets_log2"], max_chain=best["max_chain"])
+                      for _ in range(configs.BATCH_MAPS)]
+        maps = mk()
         g = BatchGraph(eng, maps, max_points=max_pts, max_scans=configs.BATCH_SCANS)
+        # warm-up on the first batch, then all other batches timed
         g.launch(descs[0], bbuf)
         eng.synchronize()
         t0 = time.perf_counter()
@@ -221,12 +223,12 @@ def main():
         tb = time.perf_counter() - t0
         bpts = sum(sum(x[1] for x in d) for d in descs[1:])
         batched_pps = bpts / tb
+        g.close()
         # the same scans through the plain per-scan path on fresh maps
         for mm in maps:
             mm.close()
-        g.close()
-        maps = [eng.map(configs.DENSE_PARAMS, buckets_log2=16) for _ in range(configs.BATCH_MAPS)]
-        j0 = len(descs[0])
+        maps = mk()
+        eng.set_launch()  # the per-scan path at its default launch shapes
         tp = 0.0
         for i, d in enumerate(descs):
             for (mi, n, off, s) in d:
@@ -241,7 +243,10 @@ def main():
                              "speedup_vs_plain": batched_pps / plain_b_pps, "launches": len(descs) - 1,
                              "scans_per_launch": configs.BATCH_SCANS, "maps": configs.BATCH_MAPS,
                              "mean_points_per_scan": bpts / ((len(descs) - 1) * configs.BATCH_SCANS),
-                             "ms_per_launch": 1e3 * tb / (len(descs) - 1)}
+                             "ms_per_launch": 1e3 * tb / (len(descs) - 1),
+                             "autotune": {"best": best, "seconds": t_tune, "sample": "first 4 batches",
+                                          "candidates": [dict(c, points_per_s=r["points_per_s"],
+                                                              failures=r["failures"]) for c, r in hist]}}
         log("batched", result["batched"])
         for mm in maps:
             mm.close()

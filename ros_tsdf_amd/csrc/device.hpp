@@ -102,9 +102,10 @@ struct Dev {
   uint32_t* seg_off;         // [entries + 1] record offset of each touched block
   uint32_t* tile_sums;       // [entries / 1024 + 2]
   uint32_t* fail_list;       // [entries] inserted entries without a heap block (removed after binning)
-  int4* fold_desc;           // [6][fold_cap] per size class: {record base, records, heap block, map}
+  int4* fold_desc;           // per size class c, from fold_base[c]: {record base, records, heap block, map}
   unsigned long long* part;  // [ST_COUNT][kPartSlots] statistics partial sums
   int64_t fold_cap;
+  int64_t fold_base[6];      // first descriptor of each size class
   unsigned long long* seg;   // [records] update records, contiguous per touched block
   int64_t seg_cap;
   uint32_t* sel;             // [entries]

@@ -45,6 +45,15 @@ constexpr int kScanTile = 1024;  // elements per workgroup in the scan kernels
 enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WG, FC_WG2, FC_HUGE, FC_COUNT };
 static_assert(FC_COUNT <= (int)(sizeof(EngCtr::fold_n) / sizeof(int32_t)), "fold class counters");
 
+// Descriptors a class can need when a pass holds at most `blocks` touched
+// blocks and `records` records: every block of class c holds at least lo
+// records (the class's lower size bound).
+__host__ __device__ constexpr int64_t fold_class_cap(int c, int64_t blocks, int64_t records) {
+  const int64_t lo = c == FC_16 ? 1 : c == FC_32 ? 17 : c == FC_64 ? 33 : c == FC_WG ? 65 : c == FC_WG2 ? kCountTileS + 1 : kCountTile + 1;
+  const int64_t r = records / lo + 1;
+  return r < blocks ? r : blocks;
+}
+
 __host__ __device__ constexpr int fold_class(int n) {
   return n <= 16 ? FC_16 : n <= 32 ? FC_32 : n <= 64 ? FC_64 : n <= kCountTileS ? FC_WG : n <= kCountTile ? FC_WG2 : FC_HUGE;
 }
@@ -866,7 +875,7 @@ __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      if (cls[k] >= 0) D.fold_desc[(int64_t)cls[k] * D.fold_cap + gbase[cls[k]] + rank[k]] = dsc[k];
+      if (cls[k] >= 0) D.fold_desc[D.fold_base[cls[k]] + gbase[cls[k]] + rank[k]] = dsc[k];
     __syncthreads();
   }
   const int nf = ld_acq(&D.ctr->fail_n);
@@ -887,7 +896,7 @@ __device__ __forceinline__ void fold_small_class(const Dev& D, unsigned long lon
   const int gl = lane & (G - 1);
   const int grp = lane / G;
   const int n_desc = ld_acq(&D.ctr->fold_n[kCls]);
-  const int4* desc = D.fold_desc + (int64_t)kCls * D.fold_cap;
+  const int4* desc = D.fold_desc + D.fold_base[kCls];
   const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int nw = gridDim.x * (blockDim.x >> 6);
   for (int q0 = gw * kPer; q0 < n_desc; q0 += nw * kPer) {
@@ -1108,7 +1117,7 @@ __global__ void __launch_bounds__(256) k_fold_a(Dev D) {
   __shared__ uint32_t sh[16];
   unsigned long long uniq = 0;
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG]);
-  const int4* desc = D.fold_desc + (int64_t)FC_WG * D.fold_cap;
+  const int4* desc = D.fold_desc + D.fold_base[FC_WG];
   for (int q = blockIdx.x; q < n_wg; q += gridDim.x)
     fold_count_block<State, kCountTileS>(D, desc[q], rin, rout, cnt, start, sh, uniq);
   fold_small_class<State, 64>(D, uniq);
@@ -1129,11 +1138,11 @@ __global__ void __launch_bounds__(256) k_fold_b(Dev D) {
   __shared__ uint32_t sh[16];
   unsigned long long uniq = 0, dropped = 0;
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG2]);
-  const int4* desc = D.fold_desc + (int64_t)FC_WG2 * D.fold_cap;
+  const int4* desc = D.fold_desc + D.fold_base[FC_WG2];
   for (int q = blockIdx.x; q < n_wg; q += gridDim.x)
     fold_count_block<State, kCountTile>(D, desc[q], buf, buf + kCountTile, cnt, start, sh, uniq);
   const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
-  const int4* hdesc = D.fold_desc + (int64_t)FC_HUGE * D.fold_cap;
+  const int4* hdesc = D.fold_desc + D.fold_base[FC_HUGE];
   for (int q = blockIdx.x; q < n_huge; q += gridDim.x) fold_huge_block<State>(D, hdesc[q], buf, uniq, dropped);
   stat_add(D, ST_UNIQUE, uniq);
   stat_add(D, ST_UPD_DROPPED, dropped);

@@ -132,6 +132,7 @@ struct tsdf_engine {
   int4* fold_desc = nullptr;
   unsigned long long* part = nullptr;
   int64_t fold_cap = 0;
+  int64_t fold_base[6] = {};
   uint8_t* touched_map = nullptr;
   uint32_t* sel = nullptr;
   uint32_t* pub_cnt = nullptr;
@@ -193,6 +194,7 @@ struct tsdf_engine {
     D.fold_desc = fold_desc;
     D.part = part;
     D.fold_cap = fold_cap;
+    for (int c = 0; c < 6; ++c) D.fold_base[c] = fold_base[c];
     D.touched_map = touched_map;
     D.seg = seg;
     D.seg_cap = seg_cap;
@@ -283,8 +285,14 @@ int ensure_scratch(tsdf_engine* E, int64_t P, int stride) {
   CHK(dev_alloc(&E->seg, np * ns));
   (void)hipFree(E->fold_desc);
   E->fold_desc = nullptr;
+  // one region per size class, each as large as that class can get
   E->fold_cap = std::min<int64_t>(E->entries, np * ns);
-  CHK(dev_alloc(&E->fold_desc, 6 * E->fold_cap));
+  int64_t fold_total = 0;
+  for (int c = 0; c < FC_COUNT; ++c) {
+    E->fold_base[c] = fold_total;
+    fold_total += fold_class_cap(c, E->fold_cap, np * ns);
+  }
+  CHK(dev_alloc(&E->fold_desc, fold_total));
   E->pcap = np;
   E->stride_cap = ns;
   E->seg_cap = np * ns;

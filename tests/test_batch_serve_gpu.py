@@ -166,3 +166,64 @@ def test_fp16_state_error(oracle_lib):
         buf.free()
         m.close()
         om.close()
+
+
+def dense_scans(n_maps, per_map, n_points):
+    out = []
+    for mi in range(n_maps):
+        w = configs.map_workload(configs.DENSE, mi)
+        boxes = synth.make_boxes(w.scene, w.seed, mi)
+        for k in range(per_map):
+            p, s, _ = synth.make_scan(w, k, map_id=mi, boxes=boxes, n_points=n_points)
+            out.append((p, s, mi))
+    return out
+
+
+def test_autotune_knobs_do_not_change_results():
+    """Every launch shape and table shape the autotuner may pick integrates the
+    same scans into the same maps, bit for bit (dense scans, so every fold
+    size class is exercised)."""
+    from ros_tsdf_amd import BatchGraph, Engine, autotune
+    n_maps, per_map = 2, 3
+    scans = dense_scans(n_maps, per_map, 65536)
+    # batches: scan k of every map, scans contiguous per batch
+    order = [mi * per_map + k for k in range(per_map) for mi in range(n_maps)]
+    pool = workloads.Pool([scans[i] for i in order])
+    batches = [[(pool.maps[j], pool.counts[j], pool.offsets[j], pool.sensors[j])
+                for j in range(k * n_maps, (k + 1) * n_maps)] for k in range(per_map)]
+    params = configs.DENSE_PARAMS
+    cfgs = [dict(traverse_block=64, fold_grid=512, buckets_log2=13, max_chain=64),
+            dict(traverse_block=256, fold_grid=4096, buckets_log2=16, max_chain=64),
+            dict(traverse_block=128, fold_grid=1024, buckets_log2=15, max_chain=16)]
+    with Engine(heap_blocks=1 << 18, total_buckets=1 << 19, max_batch_points=1 << 18) as e:
+        buf = e.buffer(pool.points.nbytes)
+        buf.upload(pool.points)
+        ref = None
+        for cfg in cfgs:
+            e.set_launch(traverse_block=cfg["traverse_block"], fold_grid=cfg["fold_grid"])
+            maps = [e.map(params, buckets_log2=cfg["buckets_log2"], max_chain=cfg["max_chain"])
+                    for _ in range(n_maps)]
+            g = BatchGraph(e, maps, max_points=n_maps * 65536, max_scans=n_maps)
+            for b in batches:
+                g.launch(b, buf)
+                st = e.stats()
+                assert st["insert_failures"] == st["heap_failures"] == st["updates_dropped"] == 0, (cfg, st)
+            g.close()
+            out = [m.export() for m in maps]
+            if ref is None:
+                ref = out
+            else:
+                for (ka, sa, wa), (kb, sb, wb) in zip(ref, out):
+                    assert np.array_equal(ka, kb), cfg
+                    assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32)), cfg
+                    assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32)), cfg
+            for m in maps:
+                m.close()
+        # the tuner itself, over a two-value space: it measures, filters and picks
+        best, hist = autotune.tune(e, params, n_maps, batches, buf,
+                                   space={"traverse_block": (128, 256), "buckets_log2": (14, 15)},
+                                   start=dict(traverse_block=256, fold_grid=1024, buckets_log2=15, max_chain=64))
+        assert len(hist) >= 3 and all(r["failures"] == 0 and r["points_per_s"] > 0 for _, r in hist)
+        assert best["traverse_block"] in (128, 256) and best["buckets_log2"] in (14, 15)
+        assert e.get_launch()["traverse_block"] == best["traverse_block"]
+        buf.free()
