@@ -41,7 +41,7 @@ struct MapDev {
 // Per-map mutable counters.
 struct MapCtr {
   int32_t ovf_top;    // entries on the overflow-bucket free stack
-  int32_t orphan_n;   // overflow buckets popped by a chain extension that lost its link race
+  int32_t pad0;
   int32_t blocks;     // heap blocks held
   int32_t pad;
 };
@@ -86,7 +86,6 @@ struct Dev {
   uint32_t* touch_idx;       // [entries] index in the touched list
   int32_t* next;             // [buckets] overflow link, -1 none
   int32_t* ovf_stack;        // [buckets] per-map overflow free stacks
-  int32_t* orphans;          // [buckets] per-map orphan lists
   void* heap;                // [heap_blocks][512] voxel state
   int32_t* heap_free;        // [heap_blocks] free stack
   int32_t* freed;            // [heap_blocks] blocks freed by gc (to be zeroed)

@@ -27,7 +27,7 @@
 //
 // publish: k_select (radius test per entry) -> k_pub_count (wave per block)
 //          -> scan -> k_pub_write (wave ballot compaction)
-// gc:      k_gc_mark -> k_gc_zero -> k_gc_chains -> k_gc_orphans
+// gc:      k_gc_mark -> k_gc_zero -> k_gc_chains
 #pragma once
 #include "device.hpp"
 
@@ -124,17 +124,26 @@ __global__ void __launch_bounds__(256) k_stats_reduce(Dev D, uint32_t mask) {
 // Find-or-insert of a packed block key in the map's chain of buckets.
 // Lock-free: the whole chain is searched first; if the key is absent the
 // first empty slot (in chain order) is claimed by CAS.  Slots only go from
-// empty to full while a pass runs (deletion happens in gc, a separate pass),
-// so every inserter of one key walks the same sequence of empty slots and
-// the first CAS winner is seen by the others: no duplicates.  A full chain
-// is extended by popping an overflow bucket and CAS-ing the tail link; the
-// loser of that race files its bucket on the orphan list (recycled by gc).
-// Returns the global entry index, or -1 when the chain limit or the overflow
-// pool is exhausted.  *inserted is set when this call created the entry.
+// empty to full while binning runs (keys leave the table only afterwards:
+// in gc, and for entries that got no heap block), so every inserter of one
+// key walks the same sequence of empty slots and the first CAS winner is
+// seen by the others: no duplicates.
+// A full chain is extended at its tail: the link is claimed first (CAS -1 ->
+// kLinkPending), so exactly one inserter pops an overflow bucket and
+// publishes it (or kLinkExhausted when the pool is empty); the others wait
+// for the published link and continue there.  An inserter therefore fails
+// only when no inserter of any key could extend that chain (chain limit or
+// empty pool), never because another one was extending it at that moment.
+// Returns the global entry index, or -1 on such a failure.  *inserted is set
+// when this call created the entry.
 // Reads are plain loads: a stale read can only show an older value of a slot
 // (empty instead of filled) or of a link (none instead of set); the CAS that
 // every insert and extension goes through then fails and reports the current
 // value, so staleness costs a retry, never a duplicate.
+constexpr int kLinkNone = -1;       // chain ends here
+constexpr int kLinkPending = -2;    // an inserter is extending the chain here
+constexpr int kLinkExhausted = -3;  // the overflow pool was empty (reset by gc)
+constexpr int kLinkSpin = 1 << 16;  // bound on waiting for a pending link
 __device__ inline int find_or_insert(const Dev& D, const MapDev& m, unsigned long long key, bool& inserted) {
   inserted = false;
   int g = m.bucket_base + (int)(hash_key(key) & m.home_mask);
@@ -185,21 +194,23 @@ __device__ inline int find_or_insert(const Dev& D, const MapDev& m, unsigned lon
     }
     // no empty slot: extend the chain at its tail
     if (tail_depth >= m.max_chain) return -1;
-    MapCtr* mc = &D.mctr[m.slot];
-    const int idx = atomicSub(&mc->ovf_top, 1) - 1;
-    if (idx < 0) {
-      atomicAdd(&mc->ovf_top, 1);
-      return -1;
+    int link = atomicCAS(&D.next[tail], kLinkNone, kLinkPending);
+    if (link == kLinkNone) {
+      // this lane owns the extension: pop a bucket and publish the link
+      MapCtr* mc = &D.mctr[m.slot];
+      const int idx = atomicSub(&mc->ovf_top, 1) - 1;
+      if (idx < 0) {
+        atomicAdd(&mc->ovf_top, 1);
+        link = kLinkExhausted;
+      } else {
+        link = D.ovf_stack[m.bucket_base + m.n_home + idx];
+      }
+      atomicExch(&D.next[tail], link);
     }
-    const int nb = D.ovf_stack[m.bucket_base + m.n_home + idx];
-    const int prev = atomicCAS(&D.next[tail], -1, nb);
-    if (prev == -1) {
-      g = nb;
-    } else {
-      const int o = atomicAdd(&mc->orphan_n, 1);
-      D.orphans[m.bucket_base + m.n_home + o] = nb;
-      g = prev;
-    }
+    // another lane's extension: wait until its link is published
+    for (int w = 0; link == kLinkPending && w < kLinkSpin; ++w) link = ld_acq(&D.next[tail]);
+    if (link < 0) return -1;
+    g = link;
     s0 = 0;
     depth = tail_depth + 1;
   }
@@ -1311,6 +1322,7 @@ __global__ void __launch_bounds__(256) k_gc_chains(Dev D, int map_slot) {
   for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < m.n_home; h += gridDim.x * blockDim.x) {
     int prev = m.bucket_base + h;
     int cur = D.next[prev];
+    if (cur == kLinkExhausted) D.next[prev] = kLinkNone;  // buckets may be back in the pool
     for (int guard = 0; cur >= 0 && guard <= m.n_ovf; ++guard) {
       const int nx = D.next[cur];
       bool empty = true;
@@ -1325,21 +1337,10 @@ __global__ void __launch_bounds__(256) k_gc_chains(Dev D, int map_slot) {
       }
       cur = nx;
     }
+    if (cur == kLinkExhausted) D.next[prev] = kLinkNone;
   }
 }
 
-__global__ void k_gc_orphans(Dev D, int map_slot) {
-  const MapDev m = D.maps[map_slot];
-  MapCtr* mc = &D.mctr[map_slot];
-  const int n = mc->orphan_n;
-  for (int o = threadIdx.x; o < n; o += blockDim.x) {
-    const int b = D.orphans[m.bucket_base + m.n_home + o];
-    const int idx = atomicAdd(&mc->ovf_top, 1);
-    D.ovf_stack[m.bucket_base + m.n_home + idx] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) mc->orphan_n = 0;
-}
 
 // ---- plain device copy (also the reference point for HBM byte counters) ------------
 __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {

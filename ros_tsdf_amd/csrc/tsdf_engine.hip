@@ -107,7 +107,6 @@ struct tsdf_engine {
   uint32_t* touch_idx = nullptr;
   int32_t* next = nullptr;
   int32_t* ovf_stack = nullptr;
-  int32_t* orphans = nullptr;
   void* heap = nullptr;
   int32_t* heap_free = nullptr;
   int32_t* freed = nullptr;
@@ -177,7 +176,6 @@ struct tsdf_engine {
     D.touch_idx = touch_idx;
     D.next = next;
     D.ovf_stack = ovf_stack;
-    D.orphans = orphans;
     D.heap = heap;
     D.heap_free = heap_free;
     D.freed = freed;
@@ -546,7 +544,6 @@ int gc_impl(tsdf_map* m, const float sensor[3], int mode, int64_t* removed) {
                      sensor[2], mode);
   hipLaunchKernelGGL(k_gc_zero<State>, dim3(1024), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k_gc_chains, dim3(grid_for(m->desc.n_home, 256, 4096)), dim3(256), 0, st, D, m->slot);
-  hipLaunchKernelGGL(k_gc_orphans, dim3(1), dim3(256), 0, st, D, m->slot);
   hipLaunchKernelGGL(k_stats_reduce, dim3(ST_COUNT), dim3(256), 0, st, D, 1u << ST_GC_REMOVED);
   HIPCHK(hipGetLastError());
   CHK(timing_mark(E, tid, 1));
@@ -699,7 +696,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
   if ((r = dev_alloc(&E->maps_dev, kMaxMaps)) || (r = dev_alloc(&E->mctr, kMaxMaps)) || (r = dev_alloc(&E->ctr, 1)) ||
       (r = dev_alloc(&E->keys, entries)) || (r = dev_alloc(&E->vals, entries)) || (r = dev_alloc(&E->blk_cnt, entries)) ||
       (r = dev_alloc(&E->touch_idx, entries)) || (r = dev_alloc(&E->next, c.total_buckets)) ||
-      (r = dev_alloc(&E->ovf_stack, c.total_buckets)) || (r = dev_alloc(&E->orphans, c.total_buckets)) ||
+      (r = dev_alloc(&E->ovf_stack, c.total_buckets)) ||
       (r = dev_alloc(&E->heap_free, c.heap_blocks)) || (r = dev_alloc(&E->freed, c.heap_blocks)) ||
       (r = dev_alloc(&E->desc_dev, 1)) ||
       (r = dev_alloc(&E->touched, entries)) || (r = dev_alloc(&E->seg_off, entries + 1)) ||
@@ -760,7 +757,7 @@ int tsdf_engine_destroy(tsdf_engine* E) {
       E->map_by_slot[i] = nullptr;
     }
   void* ptrs[] = {E->maps_dev, E->mctr, E->ctr, E->keys, E->vals, E->blk_cnt, E->touch_idx, E->next, E->ovf_stack,
-                  E->orphans, E->heap, E->heap_free, E->freed, E->desc_dev, E->pair_e, E->pair_r, E->pair_n,
+                  E->heap, E->heap_free, E->freed, E->desc_dev, E->pair_e, E->pair_r, E->pair_n,
                   E->seg, E->touched, E->seg_off, E->tile_sums, E->fail_list, E->fold_desc, E->part, E->touched_map, E->sel,
                   E->pub_cnt, E->pub_off, E->pub_buf};
   for (void* p : ptrs)

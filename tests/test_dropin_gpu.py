@@ -1,0 +1,52 @@
+"""GPU: the C++ drop-in (tsdf::TSDFHandler over the C ABI) driven the way the reference's ROS node drives it.
+
+`handler_dropin_test` links `libtsdf_package.so` like the node would. It calls initGlobalVars, then
+processPointCloudAndUpdateVoxels per scan, with caller-owned PUBLISH_VOXELS_MAX_SIZE arrays and a zeroed
+count. It writes each scan's published voxels. They must equal the oracle's integrate -> publish -> gc
+sequence (H:1003-1007): positions bit-exact as sets, values within tolerance.
+"""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from ros_tsdf_amd import _build, configs, synth
+from tests.helpers import compare_publish
+
+pytestmark = pytest.mark.gpu
+
+
+def test_dropin_handler_matches_oracle(tmp_path, oracle_lib):
+    if not os.path.exists(_build.HANDLER_TEST_BIN):
+        pytest.fail("drop-in test driver %s is not built" % _build.HANDLER_TEST_BIN)
+    params = configs.SMALL_PARAMS
+    scans = [(p, s) for _, p, s in synth.scans(configs.SMALL, stop=12)]
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as f:
+        f.write(struct.pack("<i5f", len(scans), *params))
+        for p, s in scans:
+            f.write(struct.pack("<i3f", p.shape[0], *[float(x) for x in s]))
+            f.write(np.ascontiguousarray(p, np.float32).tobytes())
+    out = tmp_path / "out.bin"
+    env = dict(os.environ, TSDF_HEAP_BLOCKS="65536")
+    r = subprocess.run([_build.HANDLER_TEST_BIN, str(inp), str(out)], env=env, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = out.read_bytes()
+    om = oracle_lib.OracleMap(params)
+    T = params[1]
+    at = 0
+    for p, s in scans:
+        (n,) = struct.unpack_from("<i", raw, at)
+        at += 4
+        rec = np.frombuffer(raw, np.float32, count=5 * n, offset=at).reshape(n, 5)
+        at += 20 * n
+        om.integrate(p, s)
+        opos, osdf, ow = om.publish(s)
+        om.gc(s)
+        compare_publish(rec[:, :3], rec[:, 3:5], opos, osdf, ow, T, configs.SDF_TOL_T,
+                        configs.WEIGHT_TOL_REL * params[2])
+    assert at == len(raw)
+    om.close()

@@ -1,0 +1,276 @@
+"""GPU edge cases, each checked against the FP64 oracle or against the engine's own per-scan path.
+
+Inputs: empty and ragged scans, invalid points (NaN, infinity, zero range, beyond the coordinate limit),
+points on exact cell and block boundaries, and coordinates near the validity limit.
+
+Capacity cases: heap exhaustion, a chain limit that makes inserts fail, publish capacity truncation, the
+Voxel.msg layout, map admission against a heap budget, per-map block quotas, and a scan larger than the
+initial scratch.
+
+When blocks cannot be stored (heap, chain or quota exhaustion), the whole block's updates are dropped. Every
+block that is stored received all of its updates, so it matches the oracle's block bit-exactly on keys and
+within tolerance on values.
+"""
+import numpy as np
+import pytest
+
+from ros_tsdf_amd import abi, configs, synth, workloads
+from tests.helpers import compare_maps, compare_publish
+
+pytestmark = pytest.mark.gpu
+
+P = configs.SMALL_PARAMS      # voxel 0.5 m, truncation 0.1 m
+D = configs.DENSE_PARAMS      # voxel 0.1 m, truncation 0.4 m
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from ros_tsdf_amd import Engine
+    e = Engine(heap_blocks=1 << 16, total_buckets=1 << 18, max_batch_points=1 << 16)
+    yield e
+    e.close()
+
+
+def tol(params):
+    return configs.SDF_TOL_T, configs.WEIGHT_TOL_REL * params[2]
+
+
+def small_scan(k=0, mi=0, n=None):
+    w = configs.map_workload(configs.SMALL, mi)
+    p, s, _ = synth.make_scan(w, k, map_id=mi, n_points=n)
+    return p, s
+
+
+def dense_scan(k=0, mi=0, n=None):
+    w = configs.map_workload(configs.DENSE, mi)
+    p, s, _ = synth.make_scan(w, k, map_id=mi, n_points=n)
+    return p, s
+
+
+def as_points(xyz):
+    xyz = np.asarray(xyz, np.float32).reshape(-1, 3)
+    out = np.zeros((xyz.shape[0], 4), np.float32)
+    out[:, :3] = xyz
+    return out
+
+
+def integrate_both(eng, oracle_lib, params, scans, **map_kw):
+    m = eng.map(params, **map_kw)
+    om = oracle_lib.OracleMap(params)
+    stats = []
+    for pts, s in scans:
+        buf = eng.buffer(max(pts.nbytes, 16))
+        if pts.shape[0]:
+            buf.upload(pts)
+        m.integrate(buf, pts.shape[0], s)
+        stats.append((eng.stats(), om.integrate(pts, s)))
+        buf.free()
+    return m, om, stats
+
+
+def compare_stored_blocks(gmap, omap, T, sdf_tol_T, w_tol):
+    """Every stored block exists in the oracle and matches it; returns (stored, oracle) block counts."""
+    gk, gs, gw = gmap.export()
+    ok, os_, ow = omap.export()
+    idx = {tuple(k): i for i, k in enumerate(ok.tolist())}
+    for j, k in enumerate(gk.tolist()):
+        i = idx.get(tuple(k))
+        assert i is not None, "block %s is not in the oracle map" % (k,)
+        t = ow[i] != 0
+        assert np.array_equal(gw[j] != 0, t), "touched voxels of block %s differ" % (k,)
+        if t.any():
+            assert np.abs(gs[j][t].astype(np.float64) - os_[i][t]).max() / T <= sdf_tol_T
+            assert np.abs(gw[j][t].astype(np.float64) - ow[i][t]).max() <= w_tol
+    return gk.shape[0], ok.shape[0]
+
+
+def test_empty_scan_and_empty_batch(eng):
+    m = eng.map(P)
+    buf = eng.buffer(16)
+    m.integrate(buf, 0, (1.0, 2.0, 0.0))
+    st = eng.stats()
+    assert st["points"] == 0 and st["valid"] == 0 and st["voxel_updates"] == 0
+    assert m.num_blocks() == 0
+    pos, data, n = m.publish((1.0, 2.0, 0.0))
+    assert n == 0 and pos.shape[0] == 0
+    assert m.gc((1.0, 2.0, 0.0)) == 0
+    eng.integrate_batch([m], [], buf)
+    eng.integrate_batch([m], [(0, 0, 0, (0.0, 0.0, 0.0)), (0, 0, 0, (1.0, 0.0, 0.0))], buf)
+    eng.synchronize()
+    assert m.num_blocks() == 0
+    m.close()
+    buf.free()
+
+
+def test_invalid_points_skipped_like_the_oracle(eng, oracle_lib):
+    pts, s = small_scan(0)
+    lim = P[0] * 65536.0
+    bad = as_points([[np.nan, 1.0, 0.0], [1.0, np.inf, 0.0], [0.0, 0.0, -np.inf],
+                     [s[0], s[1], s[2]],                       # zero range
+                     [lim * 1.5, 0.0, 0.0], [0.0, -lim * 2.0, 1.0]])  # beyond the coordinate limit
+    mixed = np.concatenate([pts[:300], bad, pts[300:]])
+    m, om, stats = integrate_both(eng, oracle_lib, P, [(mixed, s)])
+    g, o = stats[0]
+    assert g["skipped"] == o["n_skipped"] == 6
+    assert g["valid"] == o["n_valid"] == pts.shape[0]
+    assert g["voxel_updates"] == o["n_voxel_updates"]
+    compare_maps(m, om, P[1], *tol(P))
+    m.close()
+    om.close()
+
+
+def test_points_on_cell_and_block_boundaries(eng, oracle_lib):
+    # exact multiples of the voxel and block sizes on every axis, sensor on a
+    # block corner: every DDA starts, ends or steps on a boundary and ties
+    vs = D[0]
+    g = np.arange(-24, 25, dtype=np.float32)
+    xs, ys = np.meshgrid(g * np.float32(vs), g * np.float32(8 * vs))
+    zs = np.full_like(xs, np.float32(-2.0))
+    pts = as_points(np.stack([xs.ravel(), ys.ravel(), zs.ravel()], 1))
+    s = np.array([0.0, 0.0, 0.0], np.float32)
+    s2 = np.array([np.float32(8 * vs) * 3, np.float32(vs) * 5, np.float32(0.8)], np.float32)
+    m, om, stats = integrate_both(eng, oracle_lib, D, [(pts, s), (pts, s2)])
+    for g_, o_ in stats:
+        assert g_["voxel_updates"] == o_["n_voxel_updates"] and g_["unique_voxels"] == o_["n_unique_voxels"]
+    compare_maps(m, om, D[1], *tol(D))
+    m.close()
+    om.close()
+
+
+def test_coordinates_near_the_validity_limit(eng, oracle_lib):
+    # the scene shifted to 0.9 x 2^16 voxels from the origin (29.5 km at 0.5 m)
+    pts, s = small_scan(1)
+    off = np.float32(0.9 * 65536 * P[0])
+    p2 = pts.copy()
+    p2[:, 0] += off
+    p2[:, 1] -= off
+    s2 = s.copy()
+    s2[0] += off
+    s2[1] -= off
+    m, om, stats = integrate_both(eng, oracle_lib, P, [(p2, s2)])
+    assert stats[0][0]["valid"] == stats[0][1]["n_valid"] == pts.shape[0]
+    compare_maps(m, om, P[1], *tol(P))
+    m.close()
+    om.close()
+
+
+def test_heap_exhaustion_keeps_complete_blocks(oracle_lib):
+    from ros_tsdf_amd import Engine
+    with Engine(heap_blocks=64, total_buckets=1 << 12, max_batch_points=1 << 14) as e:
+        pts, s = dense_scan(0, n=4096)
+        m, om, stats = integrate_both(e, oracle_lib, D, [(pts, s)], buckets_log2=8)
+        g, o = stats[0]
+        assert g["heap_failures"] > 0 and g["updates_dropped"] > 0
+        assert g["new_blocks"] == 64 and m.num_blocks() == 64 and e.free_blocks() == 0
+        stored, total = compare_stored_blocks(m, om, D[1], *tol(D))
+        assert stored == 64 < total
+        # gc from far away returns every block; the heap serves the next scan again
+        far = (1e4, 1e4, 0.0)
+        assert m.gc(far) == 64 and e.free_blocks() == 64
+        pts2, s2 = dense_scan(1, n=256)
+        buf = e.buffer(pts2.nbytes)
+        buf.upload(pts2)
+        m.integrate(buf, pts2.shape[0], s2)
+        st = e.stats()
+        assert st["heap_failures"] == 0 and st["new_blocks"] == m.num_blocks() > 0
+        buf.free()
+        m.close()
+        om.close()
+
+
+def test_chain_limit_makes_inserts_fail_cleanly(eng, oracle_lib):
+    # 2 home buckets, 2 overflow buckets, chains of at most one overflow bucket: 32 entries in all
+    pts, s = dense_scan(2, n=2048)
+    m, om, stats = integrate_both(eng, oracle_lib, D, [(pts, s)], buckets_log2=1, overflow_buckets=2, max_chain=1)
+    g, o = stats[0]
+    assert g["insert_failures"] > 0 and g["updates_dropped"] > 0
+    stored, total = compare_stored_blocks(m, om, D[1], *tol(D))
+    assert 0 < stored <= 32 < total
+    m.close()
+    om.close()
+
+
+def test_publish_capacity_and_msg_layout(eng, oracle_lib):
+    pts, s = dense_scan(3, n=8192)
+    m, om, _ = integrate_both(eng, oracle_lib, D, [(pts, s)])
+    pos, data, n = m.publish(s)
+    opos, osdf, ow = om.publish(s)
+    compare_publish(pos, data, opos, osdf, ow, D[1], *tol(D))
+    cap = n // 3
+    pos3, data3, n3 = m.publish(s, capacity=cap)
+    assert n3 == n and pos3.shape[0] == cap
+    full = {tuple(r) for r in np.concatenate([pos, data], 1).view(np.uint32).tolist()}
+    part = {tuple(r) for r in np.concatenate([pos3, data3], 1).view(np.uint32).tolist()}
+    assert part <= full and len(part) == cap
+    msg, nm = m.publish_msg(s)
+    assert nm == n
+    # Voxel.msg field order: sdf, weight, x, y, z
+    as_msg = np.concatenate([data, pos], 1)
+    a = np.lexsort(as_msg.T[::-1])
+    b = np.lexsort(msg.T[::-1])
+    assert np.array_equal(as_msg[a].view(np.uint32), msg[b].view(np.uint32))
+    m.close()
+    om.close()
+
+
+def test_map_admission_budget_and_quota(oracle_lib):
+    from ros_tsdf_amd import Engine
+    with Engine(heap_blocks=1000, total_buckets=1 << 14, max_batch_points=1 << 14) as e:
+        a = e.map(D, buckets_log2=8, block_quota=600, admit=True)
+        with pytest.raises(abi.TsdfError) as ei:
+            e.map(D, buckets_log2=8, block_quota=600, admit=True)
+        assert ei.value.code == abi.TSDF_ERR_BUDGET
+        b = e.map(D, buckets_log2=8, block_quota=300, admit=True)
+        q = e.map(D, buckets_log2=8, block_quota=10)
+        pts, s = dense_scan(4, n=4096)
+        buf = e.buffer(pts.nbytes)
+        buf.upload(pts)
+        q.integrate(buf, pts.shape[0], s)
+        st = e.stats()
+        assert st["quota_failures"] > 0 and q.num_blocks() == 10
+        om = oracle_lib.OracleMap(D)
+        om.integrate(pts, s)
+        stored, total = compare_stored_blocks(q, om, D[1], *tol(D))
+        assert stored == 10 < total
+        om.close()
+        buf.free()
+        for mm in (a, b, q):
+            mm.close()
+
+
+def test_ragged_batch_equals_per_scan(eng):
+    sizes = [0, 1, 7, 1000, 0, 333, 64, 65]
+    scans = []
+    for j, n in enumerate(sizes):
+        p, s = small_scan(j, mi=j % 3, n=max(n, 1))
+        scans.append((p[:n], s, j % 3))
+    pool = workloads.Pool(scans)
+    buf = eng.buffer(max(pool.points.nbytes, 16))
+    if pool.points.shape[0]:
+        buf.upload(pool.points)
+    mb = [eng.map(P, buckets_log2=10) for _ in range(3)]
+    ms = [eng.map(P, buckets_log2=10) for _ in range(3)]
+    descs = [(pool.maps[i], pool.counts[i], pool.offsets[i], pool.sensors[i]) for i in range(len(pool))]
+    eng.integrate_batch(mb, descs, buf)
+    for (mi, n, off, s) in descs:
+        ms[mi].integrate(buf, n, s, offset_points=off)
+    eng.synchronize()
+    for a, b in zip(mb, ms):
+        ka, sa, wa = a.export()
+        kb, sb, wb = b.export()
+        assert np.array_equal(ka, kb)
+        assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
+        assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
+    for mm in mb + ms:
+        mm.close()
+    buf.free()
+
+
+def test_scan_larger_than_initial_scratch(oracle_lib):
+    from ros_tsdf_amd import Engine
+    with Engine(heap_blocks=1 << 14, total_buckets=1 << 14, max_batch_points=1024) as e:
+        pts, s = dense_scan(5, n=20000)
+        m, om, _ = integrate_both(e, oracle_lib, D, [(pts, s)], buckets_log2=10)
+        compare_maps(m, om, D[1], *tol(D))
+        m.close()
+        om.close()
