@@ -36,3 +36,51 @@ def compare_publish(gpos, gdata, opos, osdf, ow, T, sdf_tol_T, w_tol):
     assert e.max() <= sdf_tol_T, e.max()
     we = np.abs(gdata[g, 1].astype(np.float64) - ow[o])
     assert we.max() <= w_tol, we.max()
+
+
+def load_golden():
+    """The closed-form cases of tests/golden/closed_form.json (written by tests/golden/make_golden.py)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "closed_form.json")) as f:
+        return json.load(f)
+
+
+def golden_points(scan):
+    pts = np.zeros((len(scan["points"]), 4), np.float32)
+    pts[:, :3] = np.asarray(scan["points"], np.float32)
+    return pts, np.asarray(scan["sensor"], np.float32)
+
+
+def compare_to_golden(keys, sdf, w, expected, T, sdf_tol_T, w_tol_rel):
+    """An exported map (keys [n,3], sdf/weight [n,512]) against a case's closed-form expectation."""
+    exp_blocks = {tuple(b) for b in expected["blocks"]}
+    got_blocks = {tuple(k) for k in np.asarray(keys).tolist()}
+    assert got_blocks == exp_blocks, (sorted(got_blocks), sorted(exp_blocks))
+    row = {tuple(k): i for i, k in enumerate(np.asarray(keys).tolist())}
+    touched = set()
+    for (k, S, W) in expected["voxels"]:
+        kb = tuple(v >> 3 for v in k)
+        loc = (k[0] & 7) + 8 * (k[1] & 7) + 64 * (k[2] & 7)
+        touched.add((kb, loc))
+        i = row[kb]
+        assert abs(float(sdf[i][loc]) - S) <= sdf_tol_T * T, (k, float(sdf[i][loc]), S)
+        assert abs(float(w[i][loc]) - W) <= w_tol_rel * W, (k, float(w[i][loc]), W)
+    got_touched = {(tuple(np.asarray(keys)[i].tolist()), int(l)) for i in range(len(keys))
+                   for l in np.nonzero(np.asarray(w)[i])[0]}
+    assert got_touched == touched
+
+
+def compare_publish_to_golden(pos, sdf, w, expected, T, sdf_tol_T, w_tol_rel, pos_tol=1e-5):
+    exp = expected["published"]
+    assert len(pos) == len(exp), (len(pos), len(exp))
+    if not exp:
+        return
+    e_pos = np.asarray([p for p, _, _ in exp], np.float64)
+    g = np.lexsort(np.asarray(pos, np.float64).T[::-1])
+    o = np.lexsort(e_pos.T[::-1])
+    assert np.abs(np.asarray(pos, np.float64)[g] - e_pos[o]).max() <= pos_tol
+    e_s = np.asarray([s for _, s, _ in exp])[o]
+    e_w = np.asarray([x for _, _, x in exp])[o]
+    assert np.abs(np.asarray(sdf, np.float64)[g] - e_s).max() <= sdf_tol_T * T
+    assert (np.abs(np.asarray(w, np.float64)[g] - e_w) <= w_tol_rel * e_w).all()
