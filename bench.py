@@ -323,9 +323,8 @@ def main():
         sbuf = eng.buffer(spool.points.nbytes)
         sbuf.upload(spool.points)
         serving = {}
-        for label, counts in (("loguniform", "loguniform"), ("fixed131072", 131072)):
-            reqs_raw = workloads.poisson_requests(args.serve_requests, configs.SERVE_RATE, configs.SERVE_MAPS,
-                                                  counts=counts, scans_per_map=24)
+        def serve_run(counts, rate, n_req):
+            reqs_raw = workloads.poisson_requests(n_req, rate, configs.SERVE_MAPS, counts=counts, scans_per_map=24)
             reqs = []
             for (t, mi, k, n) in reqs_raw:
                 j = mi * 24 + k
@@ -342,11 +341,24 @@ def main():
                              "batches": int(st["batches"]), "requests": int(st["requests"])}
                 for mm in maps:
                     mm.close()
-            out["rate_scans_per_s"] = configs.SERVE_RATE
+            out["rate_scans_per_s"] = rate
             out["maps"] = configs.SERVE_MAPS
-            serving[label] = out
-            log("serving", label, out)
+            return out
+
+        for label, counts in (("loguniform", "loguniform"), ("fixed131072", 131072)):
+            serving[label] = serve_run(counts, configs.SERVE_RATE, args.serve_requests)
+            log("serving", label, serving[label])
+        # offered load beyond what one scan per launch sustains: continuous batching keeps up
+        sweep = []
+        for counts in ("loguniform", 131072):
+            for rate in (2000, 5000, 10000, 20000):
+                r = serve_run(counts, rate, 1000)
+                r["counts"] = counts
+                sweep.append(r)
+                log("serving sweep", counts, rate, {m: (round(r[m]["p50_ms"], 3), round(r[m]["p99_ms"], 3),
+                                                        "%.3g" % r[m]["points_per_s"]) for m in ("served", "plain")})
         result["serving"] = serving
+        result["serving_sweep"] = sweep
         sbuf.free()
         eng.close()
     else:

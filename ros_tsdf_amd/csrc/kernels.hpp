@@ -983,12 +983,35 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
   const int c = d.y;
   const unsigned long long* sg = D.seg + (uint32_t)d.x;
   for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) cnt[l] = 0;
-  __syncthreads();
-  for (int x = threadIdx.x; x < c; x += blockDim.x) {
-    const unsigned long long r = sg[x];
-    rin[x] = r;
-    atomicAdd(&cnt[rec_voxel(r)], 1u);
+  // all of this thread's records are requested before any is used (one
+  // memory latency per block, not one per record)
+  constexpr int kPer = TILE / 256;
+  unsigned long long rr[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int x = threadIdx.x + 256 * u;
+    rr[u] = x < c ? sg[x] : 0ull;
   }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int x = threadIdx.x + 256 * u;
+    if (x < c) {
+      rin[x] = rr[u];
+      atomicAdd(&cnt[rec_voxel(rr[u])], 1u);
+    }
+  }
+  __syncthreads();
+  // the states of this thread's two voxels (l and l + 256) are loaded now,
+  // so that the loads overlap the sort below
+  static_assert(kVoxelsPerBlock == 512, "two voxels per thread of 256");
+  const MapDev& m = D.maps[d.w];
+  const float maxW = m.maxW, Tt = m.T, invT = 1.0f / m.T;
+  const int64_t v0 = (int64_t)d.z * kVoxelsPerBlock + threadIdx.x, v1 = v0 + 256;
+  const bool h0 = cnt[threadIdx.x] != 0, h1 = cnt[threadIdx.x + 256] != 0;
+  float S0 = 0.f, W0 = 0.f, S1 = 0.f, W1 = 0.f;
+  if (h0) State::load(D.heap, v0, invT, Tt, S0, W0);
+  if (h1) State::load(D.heap, v1, invT, Tt, S1, W1);
   __syncthreads();
   {
     const int l0 = 2 * threadIdx.x;
@@ -1011,29 +1034,44 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
   // point order inside each voxel: a record's place is the number of records
   // of its voxel that sort before it (records of one voxel are distinct, a
   // walk visits a cell once)
+  // (eight independent LDS reads per step keep the loop from waiting on
+  // each read in turn)
   for (int x = threadIdx.x; x < c; x += blockDim.x) {
     const unsigned long long r = rout[x];
     const int l = rec_voxel(r);
     const int b0 = (int)start[l], b1 = (int)start[l + 1];
     int rank = 0;
-    for (int j = b0; j < b1; ++j) rank += rout[j] < r;
+    int j = b0;
+    for (; j + 8 <= b1; j += 8) {
+      unsigned long long v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = rout[j + u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) rank += v[u] < r ? 1 : 0;
+    }
+    for (; j < b1; ++j) rank += rout[j] < r ? 1 : 0;
     rin[b0 + rank] = r;
   }
   __syncthreads();
-  const MapDev& m = D.maps[d.w];
-  const float maxW = m.maxW, Tt = m.T, invT = 1.0f / m.T;
-  for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) {
-    const int b0 = (int)start[l], b1 = (int)start[l + 1];
-    if (b1 == b0) continue;
-    const int64_t vi = (int64_t)d.z * kVoxelsPerBlock + l;
-    float S, W;
-    State::load(D.heap, vi, invT, Tt, S, W);
-    for (int i = b0; i < b1; ++i) {
-      float w, wd;
-      weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
-      fold_step(S, W, w, wd, maxW);
-    }
-    State::store(D.heap, vi, invT, Tt, S, W);
+  const int l0 = threadIdx.x, l1 = threadIdx.x + 256;
+  const int a0 = (int)start[l0], e0 = (int)start[l0 + 1];
+  const int a1 = (int)start[l1], e1 = (int)start[l1 + 1];
+  for (int i = a0; i < e0; ++i) {
+    float w, wd;
+    weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
+    fold_step(S0, W0, w, wd, maxW);
+  }
+  for (int i = a1; i < e1; ++i) {
+    float w, wd;
+    weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
+    fold_step(S1, W1, w, wd, maxW);
+  }
+  if (h0) {
+    State::store(D.heap, v0, invT, Tt, S0, W0);
+    uniq++;
+  }
+  if (h1) {
+    State::store(D.heap, v1, invT, Tt, S1, W1);
     uniq++;
   }
   __syncthreads();
