@@ -29,6 +29,7 @@ struct MapDev {
   float vs, T, maxW, pubD2, gcD2;
   float half_vs, BL, half_BL;
   float sc_v, sc_b;       // 1/voxel_size and 1/block_size, as getVolumeCenterFromPoint computes them
+  float isc_v, isc_b;     // 1/sc_v and 1/sc_b when division by them may use div_by_scale's product form, else 0
   int32_t max_vox, max_blk;
   uint32_t home_mask;     // home buckets - 1 (power of two)
   int32_t bucket_base;    // global index of home bucket 0

@@ -40,6 +40,18 @@ __device__ __forceinline__ float cell_center(float x, float L, int32_t& k) {
   return f / scale + half;
 }
 
+// x / sc for a cell scale sc = fl(1/L): correctly rounded, like the
+// reference's division.  With isc = fl(1/sc) the quotient is formed by a
+// product and one FMA correction (Markstein); the host enables that form for
+// a map only after checking it against true division for every integer cell
+// index the validity rule allows (isc = 0 selects the division itself).
+__device__ __forceinline__ float div_by_scale(float x, float sc, float isc) {
+  if (isc == 0.0f) return x / sc;
+  const float q0 = x * isc;
+  const float r = __builtin_fmaf(-q0, sc, x);
+  return __builtin_fmaf(r, isc, q0);
+}
+
 // Point validity (the engine's rule for inputs the reference cannot handle:
 // H:168 never terminates on NaN, H:119 divides by a zero range).
 __device__ __forceinline__ bool point_valid(const float p[3], const float s[3], float vs) {
@@ -94,8 +106,8 @@ __device__ __forceinline__ void truncation_segment(const float p[3], const float
 // emit(k0, k1, k2) receives each cell and returns false to stop the walk
 // early (the caller has seen every cell it needs).
 template <class Emit>
-__device__ __forceinline__ int walk_cells(const float a[3], const float b[3], float L, float scale, float half,
-                                          int max_cells, bool& trunc, Emit&& emit) {
+__device__ __forceinline__ int walk_cells(const float a[3], const float b[3], float L, float scale, float iscale,
+                                          float half, int max_cells, bool& trunc, Emit&& emit) {
   trunc = false;
   const float v[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
   float step[3], tMax[3], tDelta[3], cur[3];
@@ -104,7 +116,7 @@ __device__ __forceinline__ int walk_cells(const float a[3], const float b[3], fl
   for (int i = 0; i < 3; ++i) {
     step[i] = v[i] > 0 ? L : -1.0f * L;
     const float f = floorf(a[i] * scale);
-    const float c0 = f / scale + half;
+    const float c0 = div_by_scale(f, scale, iscale) + half;
     tMax[i] = fabsf(v[i] < 0 ? (c0 - half - a[i]) / v[i] : (c0 + half - a[i]) / v[i]);
     tDelta[i] = fabsf(L / v[i]);
     cur[i] = a[i];
@@ -120,23 +132,21 @@ __device__ __forceinline__ int walk_cells(const float a[3], const float b[3], fl
     const bool go = emit(k[0], k[1], k[2]);
     ++n;
     if (!go) return n;
-    bool sx = false, sy = false, sz = false;
-    if (tMax[0] < tMax[1]) {
-      if (tMax[0] < tMax[2]) sx = true;
-      else if (tMax[0] > tMax[2]) sz = true;
-      else { sx = true; sz = true; }
-    } else if (tMax[0] > tMax[1]) {
-      if (tMax[1] < tMax[2]) sy = true;
-      else if (tMax[1] > tMax[2]) sz = true;
-      else { sy = true; sz = true; }
-    } else {
-      if (tMax[2] < tMax[0]) sz = true;
-      else if (tMax[2] > tMax[0]) { sx = true; sy = true; }
-      else { sx = true; sy = true; sz = true; }
-    }
-    if (sx) { cur[0] += step[0]; tMax[0] += tDelta[0]; }
-    if (sy) { cur[1] += step[1]; tMax[1] += tDelta[1]; }
-    if (sz) { cur[2] += step[2]; tMax[2] += tDelta[2]; }
+    // The reference's comparison tree (H:175-227), written without branches:
+    //   t0 < t1:  x if t0 < t2, z if t0 > t2, else x and z
+    //   t0 > t1:  y if t1 < t2, z if t1 > t2, else y and z
+    //   else:     z if t2 < t0, x and y if t2 > t0, else all three
+    // ("else" also covers comparisons with NaN, which are all false).
+    const bool A = tMax[0] < tMax[1], B = tMax[0] > tMax[1], E = !A && !B;
+    const bool sx = (A && !(tMax[0] > tMax[2])) || (E && !(tMax[2] < tMax[0]));
+    const bool sy = (B && !(tMax[1] > tMax[2])) || (E && !(tMax[2] < tMax[0]));
+    const bool sz = (A && !(tMax[0] < tMax[2])) || (B && !(tMax[1] < tMax[2])) || (E && !(tMax[2] > tMax[0]));
+    cur[0] = sx ? cur[0] + step[0] : cur[0];
+    tMax[0] = sx ? tMax[0] + tDelta[0] : tMax[0];
+    cur[1] = sy ? cur[1] + step[1] : cur[1];
+    tMax[1] = sy ? tMax[1] + tDelta[1] : tMax[1];
+    cur[2] = sz ? cur[2] + step[2] : cur[2];
+    tMax[2] = sz ? tMax[2] + tDelta[2] : tMax[2];
     int32_t kn[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) kn[i] = (int32_t)floorf(cur[i] * scale);
@@ -156,8 +166,8 @@ __device__ __forceinline__ int walk_cells(const float a[3], const float b[3], fl
 
 // Centre of cell k: getVolumeCenterFromPoint (H:76-85) with floor(x * s)
 // replaced by the k it produced.
-__device__ __forceinline__ float cell_centre_of(int32_t k, float scale, float half) {
-  return (float)k / scale + half;
+__device__ __forceinline__ float cell_centre_of(int32_t k, float scale, float iscale, float half) {
+  return div_by_scale((float)k, scale, iscale) + half;
 }
 
 // Block and local index of voxel k.  The reference derives them from the
@@ -171,10 +181,14 @@ __device__ __forceinline__ int local_index(const int32_t k[3]) {
 }
 
 // getDistanceUpdate (H:562-573): |p - x|, negated when (p - s).(p - x) < 0.
+// The sign is a discrete decision and is evaluated exactly as the reference
+// writes it; the magnitude is a value (compared within a tolerance), so it
+// takes the hardware square root (1 ulp) instead of the correctly rounded
+// sequence.
 __device__ __forceinline__ float signed_distance(const float p[3], const float s[3], const float x[3]) {
   const float ps0 = p[0] - s[0], ps1 = p[1] - s[1], ps2 = p[2] - s[2];
   const float px0 = p[0] - x[0], px1 = p[1] - x[1], px2 = p[2] - x[2];
-  const float mag = sqrtf(px0 * px0 + px1 * px1 + px2 * px2);
+  const float mag = __builtin_amdgcn_sqrtf(px0 * px0 + px1 * px1 + px2 * px2);
   const float dot = ps0 * px0 + ps1 * px1 + ps2 * px2;
   return dot < 0 ? -mag : mag;
 }

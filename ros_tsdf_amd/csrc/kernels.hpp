@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
         float a[3], b[3];
         truncation_segment(p, s, m.T, a, b);
         bool tb = false;
-        const int nb = walk_cells(a, b, m.BL, m.sc_b, m.half_BL, m.max_blk, tb, [&](int32_t k0, int32_t k1, int32_t k2) {
+        const int nb = walk_cells(a, b, m.BL, m.sc_b, m.isc_b, m.half_BL, m.max_blk, tb, [&](int32_t k0, int32_t k1, int32_t k2) {
           const int32_t kb[3] = {k0, k1, k2};
           if (lds_find_or_add(tkey, lds_block_key(sd.map, kb)) < 0) {
             bool ins;
@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
           fb[2] = cb[2];
         };
         bool tv = false;
-        walk_cells(a, b, m.vs, m.sc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
+        walk_cells(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
           const int32_t kb[3] = {block_of(k0), block_of(k1), block_of(k2)};
           if (rn == 0) {
             fb[0] = pb[0] = kb[0];
@@ -765,7 +765,7 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
     // recomputed only when its index moves
     int32_t kc[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
     float vc[3] = {0.f, 0.f, 0.f};
-    walk_cells(a, b, m.vs, m.sc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
+    walk_cells(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
       const unsigned long long key = pack_key(block_of(k0), block_of(k1), block_of(k2));
       if (key != rkey) {
         rkey = key;
@@ -781,7 +781,7 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
         for (int ax = 0; ax < 3; ++ax)
           if (kk[ax] != kc[ax]) {
             kc[ax] = kk[ax];
-            vc[ax] = cell_centre_of(kk[ax], m.sc_v, m.half_vs);
+            vc[ax] = cell_centre_of(kk[ax], m.sc_v, m.isc_v, m.half_vs);
           }
         D.seg[at++] = make_rec(local_index(kk), i, signed_distance(p, s, vc));
       }
@@ -809,14 +809,16 @@ __device__ __forceinline__ int next_pow2(int c) {
 }
 
 // H:637-654 with the update's weight w = 1/(|d|+1) and w*d precomputed:
-// S = (W*S + w*d)/(W+w); W = min(W+w, maxW).
+// S = (W*S + w*d)/(W+w); W = min(W+w, maxW).  These are values (the FP64
+// oracle bounds them within a tolerance), so the quotients use the hardware
+// reciprocal (1 ulp) rather than the correctly rounded division sequence.
 __device__ __forceinline__ void fold_step(float& S, float& W, float w, float wd, float maxW) {
   const float Wn = W + w;
-  S = (W * S + wd) / Wn;
+  S = (W * S + wd) * __builtin_amdgcn_rcpf(Wn);
   W = fminf(Wn, maxW);
 }
 __device__ __forceinline__ void weight_of(float d, float& w, float& wd) {
-  w = 1.0f / (fabsf(d) + 1.0f);
+  w = __builtin_amdgcn_rcpf(fabsf(d) + 1.0f);
   wd = w * d;
 }
 

@@ -220,6 +220,21 @@ int set_device(tsdf_engine* E) {
   return TSDF_OK;
 }
 
+// 1/sc when k / sc == fma(fma(-q0, sc, k), 1/sc, q0), q0 = k * (1/sc), for
+// every integer k a valid point can produce (|k| <= 2^17, with margin),
+// else 0: the map then divides.
+float product_form_divisor(float sc) {
+  const float isc = 1.0f / sc;
+  if (!std::isfinite(isc) || isc == 0.0f) return 0.0f;
+  for (int k = -(1 << 18); k <= (1 << 18); ++k) {
+    const float x = (float)k;
+    const float q0 = x * isc;
+    const float r = std::fmaf(-q0, sc, x);
+    if (std::fmaf(r, isc, q0) != x / sc) return 0.0f;
+  }
+  return isc;
+}
+
 void derive(const tsdf_params& p, MapDev& m) {
   // initGlobalVars, H:1289-1317
   m.vs = p.voxel_size;
@@ -232,6 +247,8 @@ void derive(const tsdf_params& p, MapDev& m) {
   m.half_BL = m.BL / 2;
   m.sc_v = 1 / m.vs;
   m.sc_b = 1 / m.BL;
+  m.isc_v = product_form_divisor(m.sc_v);
+  m.isc_b = product_form_divisor(m.sc_b);
   m.max_vox = (int)(std::ceil(p.truncation_distance * 2 / p.voxel_size) * 4);
   m.max_blk = (int)(std::ceil(p.truncation_distance * 2 / m.BL) * 4);
 }
