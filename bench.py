@@ -253,21 +253,24 @@ def main():
         bbuf.free()
         eng.close()
 
-        # ---------------- FP16 voxel state ----------------
-        eng = Engine(heap_blocks=1 << 20, total_buckets=1 << 21, max_batch_points=1 << 17,
-                     state_format=abi.TSDF_STATE_F16)
-        mh = eng.map(configs.DENSE_PARAMS, buckets_log2=17)
-        hbuf = eng.buffer(dpool.points.nbytes)
-        hbuf.upload(dpool.points)
-        plain_throughput(eng, mh, hbuf, dpool, order[:W], with_stats=False)
-        hs, hn, _ = plain_throughput(eng, mh, hbuf, dpool, order[W:], with_stats=False)
-        result["fp16"] = {"points_per_s": hn / hs, "bytes_per_voxel": 4, "bytes_per_voxel_update_rw": 8}
-        mh.close()
-        # error after error-scans scans, both formats, against the oracle
+        # ---------------- packed voxel state (4 bytes) ----------------
+        # "fp16": FP16 sdf/T + FP16 weight; "fp16_qw": FP16 sdf/T + 16-bit fixed-point weight
+        packed = {}
+        for name, fmt in (("fp16", abi.TSDF_STATE_F16), ("fp16_qw", abi.TSDF_STATE_F16_QW)):
+            e = Engine(heap_blocks=1 << 20, total_buckets=1 << 21, max_batch_points=1 << 17, state_format=fmt)
+            mp = e.map(configs.DENSE_PARAMS, buckets_log2=17)
+            pb = e.buffer(dpool.points.nbytes)
+            pb.upload(dpool.points)
+            plain_throughput(e, mp, pb, dpool, order[:W], with_stats=False)
+            hs, hn, _ = plain_throughput(e, mp, pb, dpool, order[W:], with_stats=False)
+            mp.close()
+            packed[name] = {"engine": e, "buf": pb, "map": e.map(configs.DENSE_PARAMS, buckets_log2=17),
+                            "result": {"points_per_s": hn / hs, "bytes_per_voxel": 4, "bytes_per_voxel_update_rw": 8}}
+        # error after the first E dense scans, every format, against the oracle (whose multithreaded run
+        # over the same scans is also the CPU baseline)
         E = min(args.error_scans, need)
         import oracle
         om = oracle.OracleMap(configs.DENSE_PARAMS)
-        mh = eng.map(configs.DENSE_PARAMS, buckets_log2=17)
         eng32 = Engine(heap_blocks=1 << 20, total_buckets=1 << 21, max_batch_points=1 << 17)
         m32 = eng32.map(configs.DENSE_PARAMS, buckets_log2=17)
         buf32 = eng32.buffer(dpool.points.nbytes)
@@ -277,7 +280,8 @@ def main():
         cpu_pts = 0
         cpu_scans = 0
         for k in range(E):
-            mh.integrate(hbuf, dpool.counts[k], dpool.sensors[k], offset_points=dpool.offsets[k])
+            for v in packed.values():
+                v["map"].integrate(v["buf"], dpool.counts[k], dpool.sensors[k], offset_points=dpool.offsets[k])
             m32.integrate(buf32, dpool.counts[k], dpool.sensors[k], offset_points=dpool.offsets[k])
             t0 = time.perf_counter()
             om.integrate(dense[k][0], dense[k][1], threads=nthreads)
@@ -289,28 +293,32 @@ def main():
         ok, os_, ow = om.export()
         T = configs.DENSE_PARAMS[1]
         errs = {}
-        for name, mm in (("fp16", mh), ("fp32", m32)):
+        for name, mm in [(n, v["map"]) for n, v in packed.items()] + [("fp32", m32)]:
             gk, gs, gw = mm.export()
             same = gk.shape == ok.shape and np.array_equal(gk, ok)
             t = ow != 0
             e = np.abs(gs[t].astype(np.float64) - os_[t]) / T if same else np.array([np.nan])
             we = np.abs(gw[t].astype(np.float64) - ow[t]) if same else np.array([np.nan])
+            wr = we / ow[t] if same else np.array([np.nan])
             errs[name] = {"keys_bit_exact": bool(same), "touched_bit_exact": bool(same and np.array_equal(gw != 0, t)),
                           "max_sdf_err_T": float(e.max()), "rms_sdf_err_T": float(np.sqrt(np.mean(e * e))),
-                          "max_weight_err": float(we.max()), "scans": E}
-        result["fp16"]["error_vs_oracle"] = errs["fp16"]
+                          "max_weight_err": float(we.max()), "rms_weight_err": float(np.sqrt(np.mean(we * we))),
+                          "max_weight_err_rel": float(wr.max()), "scans": E}
+        for name, v in packed.items():
+            result[name] = dict(v["result"], error_vs_oracle=errs[name])
         result["fp32_error_vs_oracle"] = errs["fp32"]
         log("errors", errs)
         cpu_baseline = {"value": cpu_pts / tcpu, "unit": "points/s", "cores": nthreads, "kind": "port",
                         "sample": "oracle multithreaded path, first %d dense scans (%d points), %.1f s" % (
                             cpu_scans, cpu_pts, tcpu)}
         om.close()
-        mh.close()
         m32.close()
-        hbuf.free()
         buf32.free()
         eng32.close()
-        eng.close()
+        for v in packed.values():
+            v["map"].close()
+            v["buf"].free()
+            v["engine"].close()
 
         # ---------------- serving ----------------
         budget_blocks = configs.SERVE_HEAP_BUDGET // (512 * 8)

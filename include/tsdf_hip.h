@@ -37,6 +37,7 @@ extern "C" {
 /* Voxel state formats of an engine's block heap. */
 #define TSDF_STATE_F32 0  /* {float sdf, float weight}: the reference Voxel, CH:40-46 */
 #define TSDF_STATE_F16 1  /* {half sdf/truncation, half weight} packed in 4 bytes   */
+#define TSDF_STATE_F16_QW 2 /* {half sdf/truncation, 16-bit fixed-point weight, step max_weight/65535} */
 
 typedef struct tsdf_engine tsdf_engine;
 typedef struct tsdf_map tsdf_map;

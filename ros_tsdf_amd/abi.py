@@ -20,6 +20,7 @@ TSDF_ERR_DEADLINE = -6
 
 TSDF_STATE_F32 = 0
 TSDF_STATE_F16 = 1
+TSDF_STATE_F16_QW = 2
 
 STAGES = ["traverse", "assign", "scan", "scatter", "fold", "publish", "gc", "total"]
 

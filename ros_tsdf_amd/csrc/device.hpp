@@ -130,15 +130,29 @@ __device__ __forceinline__ uint32_t hash_key(unsigned long long k) {
 }
 
 // ---- voxel state codecs -----------------------------------------------------
+// Per-map constants the packed formats need.
+struct Codec {
+  float T, inv_T;     // truncation distance and its reciprocal
+  float wq, inv_wq;   // 65535 / max_weight and its reciprocal (fixed-point weight)
+};
+__device__ __forceinline__ Codec make_codec(const MapDev& m) {
+  Codec c;
+  c.T = m.T;
+  c.inv_T = 1.0f / m.T;
+  c.wq = 65535.0f / m.maxW;
+  c.inv_wq = m.maxW / 65535.0f;
+  return c;
+}
+
+// The reference's Voxel {float sdf, float weight}, 8 bytes.
 struct StateF32 {
-  using T = float2;
   static constexpr int kBytes = 8;
-  __device__ static __forceinline__ void load(const void* heap, int64_t i, float inv_T, float Tt, float& S, float& W) {
+  __device__ static __forceinline__ void load(const void* heap, int64_t i, const Codec&, float& S, float& W) {
     const float2 v = reinterpret_cast<const float2*>(heap)[i];
     S = v.x;
     W = v.y;
   }
-  __device__ static __forceinline__ void store(void* heap, int64_t i, float inv_T, float Tt, float S, float W) {
+  __device__ static __forceinline__ void store(void* heap, int64_t i, const Codec&, float S, float W) {
     reinterpret_cast<float2*>(heap)[i] = make_float2(S, W);
   }
   __device__ static __forceinline__ bool nonzero(const void* heap, int64_t i) {
@@ -150,15 +164,38 @@ struct StateF32 {
 // FP16 SDF normalised by the truncation distance plus FP16 weight, 4 bytes.
 struct StateF16 {
   static constexpr int kBytes = 4;
-  __device__ static __forceinline__ void load(const void* heap, int64_t i, float inv_T, float Tt, float& S, float& W) {
+  __device__ static __forceinline__ void load(const void* heap, int64_t i, const Codec& c, float& S, float& W) {
     const uint32_t v = reinterpret_cast<const uint32_t*>(heap)[i];
-    S = __half2float(__ushort_as_half((unsigned short)(v & 0xFFFFu))) * Tt;
+    S = __half2float(__ushort_as_half((unsigned short)(v & 0xFFFFu))) * c.T;
     W = __half2float(__ushort_as_half((unsigned short)(v >> 16)));
   }
-  __device__ static __forceinline__ void store(void* heap, int64_t i, float inv_T, float Tt, float S, float W) {
-    const unsigned short s = __half_as_ushort(__float2half_rn(S / Tt));
+  __device__ static __forceinline__ void store(void* heap, int64_t i, const Codec& c, float S, float W) {
+    const unsigned short s = __half_as_ushort(__float2half_rn(S * c.inv_T));
     const unsigned short w = __half_as_ushort(__float2half_rn(W));
     reinterpret_cast<uint32_t*>(heap)[i] = (uint32_t)s | ((uint32_t)w << 16);
+  }
+  __device__ static __forceinline__ bool nonzero(const void* heap, int64_t i) {
+    return (reinterpret_cast<const uint32_t*>(heap)[i] >> 16) != 0u;
+  }
+  __device__ static __forceinline__ void zero(void* heap, int64_t i) { reinterpret_cast<uint32_t*>(heap)[i] = 0u; }
+};
+
+// FP16 SDF / truncation plus a 16-bit fixed-point weight: W = q * max_weight
+// / 65535, a uniform step of max_weight / 65535 (0.015 at 1000) instead of
+// FP16's 0.5 above 512.  4 bytes.
+struct StateF16Q {
+  static constexpr int kBytes = 4;
+  __device__ static __forceinline__ void load(const void* heap, int64_t i, const Codec& c, float& S, float& W) {
+    const uint32_t v = reinterpret_cast<const uint32_t*>(heap)[i];
+    S = __half2float(__ushort_as_half((unsigned short)(v & 0xFFFFu))) * c.T;
+    W = (float)(v >> 16) * c.inv_wq;
+  }
+  __device__ static __forceinline__ void store(void* heap, int64_t i, const Codec& c, float S, float W) {
+    const unsigned short s = __half_as_ushort(__float2half_rn(S * c.inv_T));
+    const float q = fminf(fmaxf(rintf(W * c.wq), 0.0f), 65535.0f);
+    // a touched voxel keeps a nonzero weight
+    const uint32_t w = W > 0.0f ? max((uint32_t)q, 1u) : 0u;
+    reinterpret_cast<uint32_t*>(heap)[i] = (uint32_t)s | (w << 16);
   }
   __device__ static __forceinline__ bool nonzero(const void* heap, int64_t i) {
     return (reinterpret_cast<const uint32_t*>(heap)[i] >> 16) != 0u;

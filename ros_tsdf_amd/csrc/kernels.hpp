@@ -937,10 +937,11 @@ __device__ __forceinline__ void fold_small_class(const Dev& D, unsigned long lon
     float w, wd;
     weight_of(__uint_as_float((uint32_t)v), w, wd);
     const MapDev& m = D.maps[d.w];
-    const float maxW = m.maxW, Tt = m.T, invT = 1.0f / m.T;
+    const float maxW = m.maxW;
+    const Codec cd = make_codec(m);
     const int64_t vi = (int64_t)d.z * kVoxelsPerBlock + rec_voxel(v);
     float S = 0.f, W = 0.f;
-    if (head) State::load(D.heap, vi, invT, Tt, S, W);
+    if (head) State::load(D.heap, vi, cd, S, W);
     int maxlen = len;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, o));
@@ -950,7 +951,7 @@ __device__ __forceinline__ void fold_small_class(const Dev& D, unsigned long lon
       if (r < len) fold_step(S, W, wr, wdr, maxW);
     }
     if (head) {
-      State::store(D.heap, vi, invT, Tt, S, W);
+      State::store(D.heap, vi, cd, S, W);
       uniq++;
     }
   }
@@ -1008,12 +1009,13 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
   // so that the loads overlap the sort below
   static_assert(kVoxelsPerBlock == 512, "two voxels per thread of 256");
   const MapDev& m = D.maps[d.w];
-  const float maxW = m.maxW, Tt = m.T, invT = 1.0f / m.T;
+  const float maxW = m.maxW;
+  const Codec cd = make_codec(m);
   const int64_t v0 = (int64_t)d.z * kVoxelsPerBlock + threadIdx.x, v1 = v0 + 256;
   const bool h0 = cnt[threadIdx.x] != 0, h1 = cnt[threadIdx.x + 256] != 0;
   float S0 = 0.f, W0 = 0.f, S1 = 0.f, W1 = 0.f;
-  if (h0) State::load(D.heap, v0, invT, Tt, S0, W0);
-  if (h1) State::load(D.heap, v1, invT, Tt, S1, W1);
+  if (h0) State::load(D.heap, v0, cd, S0, W0);
+  if (h1) State::load(D.heap, v1, cd, S1, W1);
   __syncthreads();
   {
     const int l0 = 2 * threadIdx.x;
@@ -1069,11 +1071,11 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
     fold_step(S1, W1, w, wd, maxW);
   }
   if (h0) {
-    State::store(D.heap, v0, invT, Tt, S0, W0);
+    State::store(D.heap, v0, cd, S0, W0);
     uniq++;
   }
   if (h1) {
-    State::store(D.heap, v1, invT, Tt, S1, W1);
+    State::store(D.heap, v1, cd, S1, W1);
     uniq++;
   }
   __syncthreads();
@@ -1095,7 +1097,8 @@ __device__ __forceinline__ void fold_huge_block(const Dev& D, const int4 d, unsi
   }
   unsigned long long* sg = D.seg + (uint32_t)d.x;
   const MapDev& m = D.maps[d.w];
-  const float maxW = m.maxW, Tt = m.T, invT = 1.0f / m.T;
+  const float maxW = m.maxW;
+  const Codec cd = make_codec(m);
   for (int ch = 0; ch < nch; ++ch) {
     const int len = min(kFoldCap, c - ch * kFoldCap);
     const int n2 = next_pow2(len);
@@ -1131,7 +1134,7 @@ __device__ __forceinline__ void fold_huge_block(const Dev& D, const int4 d, unsi
     if (!any) continue;
     const int64_t vi = (int64_t)slot * kVoxelsPerBlock + l;
     float S, W;
-    State::load(D.heap, vi, invT, Tt, S, W);
+    State::load(D.heap, vi, cd, S, W);
     for (;;) {
       int best = -1;
       unsigned long long bv = ~0ull;
@@ -1150,7 +1153,7 @@ __device__ __forceinline__ void fold_huge_block(const Dev& D, const int4 d, unsi
       fold_step(S, W, w, wd, maxW);
       cur[best]++;
     }
-    State::store(D.heap, vi, invT, Tt, S, W);
+    State::store(D.heap, vi, cd, S, W);
     uniq++;
   }
   __syncthreads();
@@ -1251,7 +1254,7 @@ __global__ void __launch_bounds__(256) k_pub_write(Dev D, int map_slot, const ui
   const MapDev m = D.maps[map_slot];
   const int n = ld_acq(&D.ctr->sel_n);
   const int wpb = blockDim.x >> 6;
-  const float Tt = m.T, invT = 1.0f / m.T;
+  const Codec cd = make_codec(m);
   for (int q = blockIdx.x * wpb + (threadIdx.x >> 6); q < n; q += gridDim.x * wpb) {
     const uint32_t e = D.sel[q];
     const int slot = D.vals[e];
@@ -1265,7 +1268,7 @@ __global__ void __launch_bounds__(256) k_pub_write(Dev D, int map_slot, const ui
       const int l = r * 64 + lane_id();
       const int64_t vi = (int64_t)slot * kVoxelsPerBlock + l;
       float S, W;
-      State::load(D.heap, vi, invT, Tt, S, W);
+      State::load(D.heap, vi, cd, S, W);
       const bool pred = W != 0.0f;
       const unsigned long long mask = __ballot(pred);
       const int64_t at = o + (int64_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
@@ -1297,14 +1300,14 @@ template <class State>
 __global__ void __launch_bounds__(256) k_export(Dev D, int map_slot, int32_t* keys_out, float2* vox_out) {
   const MapDev m = D.maps[map_slot];
   const int n = ld_acq(&D.ctr->sel_n);
-  const float Tt = m.T, invT = 1.0f / m.T;
+  const Codec cd = make_codec(m);
   for (int q = blockIdx.x; q < n; q += gridDim.x) {
     const uint32_t e = D.sel[q];
     const int slot = D.vals[e];
     if (threadIdx.x == 0) unpack_key(D.keys[e], keys_out + 3 * (int64_t)q);
     for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) {
       float S, W;
-      State::load(D.heap, (int64_t)slot * kVoxelsPerBlock + l, invT, Tt, S, W);
+      State::load(D.heap, (int64_t)slot * kVoxelsPerBlock + l, cd, S, W);
       vox_out[(int64_t)q * kVoxelsPerBlock + l] = make_float2(S, W);
     }
   }

@@ -78,6 +78,16 @@ int grid_for(int64_t n, int block, int cap) {
   return (int)g;
 }
 
+// Calls f with a value of the voxel-state codec type of format fmt.
+template <class F>
+auto with_state(int fmt, F&& f) {
+  switch (fmt) {
+    case TSDF_STATE_F16: return f(StateF16{});
+    case TSDF_STATE_F16_QW: return f(StateF16Q{});
+    default: return f(StateF32{});
+  }
+}
+
 }  // namespace
 
 struct tsdf_batch_graph;
@@ -400,13 +410,12 @@ int enqueue_pass(tsdf_engine* E, int64_t P, int tid) {
   hipLaunchKernelGGL(k_emit, dim3(grid_for(P, 256, 65536)), dim3(256), 0, st, D);
   CHK(timing_mark(E, tid, 4));
   const int fg = std::max(1, L.fold_grid);
-  if (E->state_format == TSDF_STATE_F16) {
-    hipLaunchKernelGGL(k_fold_a<StateF16>, dim3(fg), dim3(256), 0, st, D);
-    hipLaunchKernelGGL(k_fold_b<StateF16>, dim3(512), dim3(256), 0, st, D);
-  } else {
-    hipLaunchKernelGGL(k_fold_a<StateF32>, dim3(fg), dim3(256), 0, st, D);
-    hipLaunchKernelGGL(k_fold_b<StateF32>, dim3(512), dim3(256), 0, st, D);
-  }
+  with_state(E->state_format, [&](auto s) {
+    using State = decltype(s);
+    hipLaunchKernelGGL(k_fold_a<State>, dim3(fg), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_fold_b<State>, dim3(512), dim3(256), 0, st, D);
+    return 0;
+  });
   CHK(timing_mark(E, tid, 5));
   HIPCHK(hipGetLastError());
   return TSDF_OK;
@@ -688,7 +697,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
     if (cfg->total_buckets > 0) c.total_buckets = cfg->total_buckets;
     if (cfg->max_batch_points > 0) c.max_batch_points = cfg->max_batch_points;
   }
-  if (c.state_format != TSDF_STATE_F32 && c.state_format != TSDF_STATE_F16)
+  if (c.state_format != TSDF_STATE_F32 && c.state_format != TSDF_STATE_F16 && c.state_format != TSDF_STATE_F16_QW)
     return fail(TSDF_ERR_INVALID_ARG, "unknown state format %d", c.state_format);
   if (c.heap_blocks > (1ll << 30)) return fail(TSDF_ERR_INVALID_ARG, "heap_blocks too large");
   if (c.total_buckets * kBucket > (1ll << 31) - 1) return fail(TSDF_ERR_INVALID_ARG, "total_buckets too large");
@@ -722,7 +731,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
       (r = dev_alloc(&E->sel, entries)) || (r = dev_alloc(&E->pub_cnt, entries + 1)) ||
       (r = dev_alloc(&E->pub_off, entries + 1)))
     return cleanup(r);
-  const int64_t vbytes = (int64_t)c.heap_blocks * kVoxelsPerBlock * (c.state_format == TSDF_STATE_F16 ? 4 : 8);
+  const int64_t vbytes = (int64_t)c.heap_blocks * kVoxelsPerBlock * (c.state_format == TSDF_STATE_F32 ? 8 : 4);
   {
     unsigned char* h = nullptr;
     if ((r = dev_alloc(&h, vbytes))) return cleanup(r);
@@ -825,7 +834,7 @@ int tsdf_map_destroy(tsdf_map* m) {
   CHK(check_map(m));
   tsdf_engine* E = m->eng;
   const float z[3] = {0, 0, 0};
-  int r = E->state_format == TSDF_STATE_F16 ? gc_impl<StateF16>(m, z, 1, nullptr) : gc_impl<StateF32>(m, z, 1, nullptr);
+  int r = with_state(E->state_format, [&](auto s) { return gc_impl<decltype(s)>(m, z, 1, nullptr); });
   if (r) return r;
   HIPCHK(hipStreamSynchronize(E->stream));
   release_buckets(E, m->desc.bucket_base, (int64_t)m->desc.n_home + m->desc.n_ovf);
@@ -870,10 +879,10 @@ int tsdf_map_export(tsdf_map* m, int32_t* keys, float* voxels, int64_t capacity,
     hipFree(dk);
     return r;
   }
-  if (E->state_format == TSDF_STATE_F16)
-    hipLaunchKernelGGL(k_export<StateF16>, dim3(grid_for(n, 1, 8192)), dim3(256), 0, st, D, m->slot, dk, dv);
-  else
-    hipLaunchKernelGGL(k_export<StateF32>, dim3(grid_for(n, 1, 8192)), dim3(256), 0, st, D, m->slot, dk, dv);
+  with_state(E->state_format, [&](auto s) {
+    hipLaunchKernelGGL(k_export<decltype(s)>, dim3(grid_for(n, 1, 8192)), dim3(256), 0, st, D, m->slot, dk, dv);
+    return 0;
+  });
   hipError_t e1 = hipGetLastError();
   if (e1 == hipSuccess && keys) e1 = hipMemcpyAsync(keys, dk, sizeof(int32_t) * 3 * w, hipMemcpyDeviceToHost, st);
   if (e1 == hipSuccess && voxels)
@@ -909,23 +918,21 @@ int tsdf_integrate_batch(tsdf_engine* E, tsdf_map* const* maps, int32_t nmaps, c
 int tsdf_publish(tsdf_map* m, const float sensor[3], float* pos, float* data, int64_t capacity, int64_t* count) {
   CHK(check_map(m));
   if (!sensor) return fail(TSDF_ERR_INVALID_ARG, "null sensor");
-  return m->eng->state_format == TSDF_STATE_F16 ? publish_impl<StateF16>(m, sensor, pos, data, capacity, count, 0)
-                                                : publish_impl<StateF32>(m, sensor, pos, data, capacity, count, 0);
+  return with_state(m->eng->state_format,
+                    [&](auto s) { return publish_impl<decltype(s)>(m, sensor, pos, data, capacity, count, 0); });
 }
 
 int tsdf_publish_msg(tsdf_map* m, const float sensor[3], float* voxels_msg, int64_t capacity, int64_t* count) {
   CHK(check_map(m));
   if (!sensor) return fail(TSDF_ERR_INVALID_ARG, "null sensor");
-  return m->eng->state_format == TSDF_STATE_F16
-             ? publish_impl<StateF16>(m, sensor, voxels_msg, nullptr, capacity, count, 1)
-             : publish_impl<StateF32>(m, sensor, voxels_msg, nullptr, capacity, count, 1);
+  return with_state(m->eng->state_format,
+                    [&](auto s) { return publish_impl<decltype(s)>(m, sensor, voxels_msg, nullptr, capacity, count, 1); });
 }
 
 int tsdf_gc(tsdf_map* m, const float sensor[3], int64_t* removed) {
   CHK(check_map(m));
   if (!sensor) return fail(TSDF_ERR_INVALID_ARG, "null sensor");
-  return m->eng->state_format == TSDF_STATE_F16 ? gc_impl<StateF16>(m, sensor, 0, removed)
-                                                : gc_impl<StateF32>(m, sensor, 0, removed);
+  return with_state(m->eng->state_format, [&](auto s) { return gc_impl<decltype(s)>(m, sensor, 0, removed); });
 }
 
 int tsdf_process_scan(tsdf_map* m, const float* points_host, int64_t n, const float sensor[3], float* pos, float* data,

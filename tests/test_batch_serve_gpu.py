@@ -139,11 +139,12 @@ def test_serving_alone_vs_mixed(eng, oracle_lib):
     buf.free()
 
 
-def test_fp16_state_error(oracle_lib):
+@pytest.mark.parametrize("fmt", [abi.TSDF_STATE_F16, abi.TSDF_STATE_F16_QW], ids=["fp16", "fp16_fixed_weight"])
+def test_fp16_state_error(oracle_lib, fmt):
     from ros_tsdf_amd import Engine
     params = configs.DENSE_PARAMS
     with Engine(heap_blocks=1 << 16, total_buckets=1 << 18, max_batch_points=1 << 17,
-                state_format=abi.TSDF_STATE_F16) as e16:
+                state_format=fmt) as e16:
         m = e16.map(params, buckets_log2=15)
         om = oracle_lib.OracleMap(params)
         buf = e16.buffer(131072 * 16)
@@ -151,18 +152,28 @@ def test_fp16_state_error(oracle_lib):
             buf.upload(pts)
             m.integrate(buf, pts.shape[0], s)
             om.integrate(pts, s, threads=8)
-        # FP16 sdf/T has a half-ulp of 2^-11 on [0.5, 1): tolerance 1e-3 T;
-        # FP16 weights carry 11 significant bits: tolerance 1 % of the weight
+        # FP16 sdf/T has a half-ulp of 2^-11 on [0.5, 1): tolerance 1e-3 T.
+        # FP16 weights carry 11 significant bits: tolerance 1 % of the weight.
+        # Fixed-point weights have a step of max_weight / 65535 (0.0153 at
+        # 1000), half of it lost per store: tolerance 2 stores' worth.  That
+        # step is coarse for the first updates of a voxel (W ~ 1), and the
+        # next fold's sdf moves by w |S - d| / (W + w)^2 per unit of weight
+        # error (|S - d| < 2.5 T here): sdf tolerance 1e-2 T for that format.
         gk, gs, gw = m.export()
         ok, os_, ow = om.export()
         assert np.array_equal(gk, ok)
         t = ow != 0
         assert np.array_equal(gw != 0, t)
         e = np.abs(gs[t] - os_[t]) / params[1]
-        we = np.abs(gw[t] - ow[t]) / ow[t]
-        print("fp16 max %.3e rms %.3e T; weight rel max %.3e" % (e.max(), np.sqrt(np.mean(e * e)), we.max()))
-        assert e.max() <= 1e-3
-        assert we.max() <= 1e-2
+        we = np.abs(gw[t] - ow[t])
+        print("max %.3e rms %.3e T; weight abs max %.3e rel max %.3e" % (
+            e.max(), np.sqrt(np.mean(e * e)), we.max(), (we / ow[t]).max()))
+        if fmt == abi.TSDF_STATE_F16:
+            assert e.max() <= 1e-3
+            assert (we / ow[t]).max() <= 1e-2
+        else:
+            assert e.max() <= 1e-2
+            assert we.max() <= 2 * 0.5 * params[2] / 65535 + 1e-4
         buf.free()
         m.close()
         om.close()
