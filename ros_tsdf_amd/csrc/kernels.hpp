@@ -1205,30 +1205,46 @@ __global__ void __launch_bounds__(256) k_fold_b(Dev D) {
 // ---- publish ---------------------------------------------------------------------------
 // Selects the map's occupied entries; mode 0: all, 1: within the publish
 // radius of the sensor (H:812).
+// Each workgroup takes kSelChunk consecutive entries, gathers its selection
+// in LDS and appends it with one atomic (a per-wave append would put tens of
+// thousands of atomics on one counter).
+constexpr int kSelChunk = 2048;
+
 __global__ void __launch_bounds__(256) k_select(Dev D, int map_slot, float sx, float sy, float sz, int mode) {
+  __shared__ uint32_t lsel[kSelChunk];
+  __shared__ int ln, lbase;
   const MapDev m = D.maps[map_slot];
   const int64_t e0 = (int64_t)m.bucket_base * kBucket;
   const int64_t ne = (int64_t)(m.n_home + m.n_ovf) * kBucket;
   const float s[3] = {sx, sy, sz};
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < ne; base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t x = base + threadIdx.x;
-    bool pred = false;
-    if (x < ne) {
-      const unsigned long long k = D.keys[e0 + x];
-      if (k != kEmptyKey && D.vals[e0 + x] >= 0) {
-        if (mode == 0) {
-          pred = true;
-        } else {
-          int32_t kb[3];
-          unpack_key(k, kb);
-          float c[3];
-          block_center(kb, m.BL, c);
-          pred = dist2(c, s) <= m.pubD2;
+  for (int64_t c0 = (int64_t)blockIdx.x * kSelChunk; c0 < ne; c0 += (int64_t)gridDim.x * kSelChunk) {
+    if (threadIdx.x == 0) ln = 0;
+    __syncthreads();
+    for (int u = 0; u < kSelChunk; u += 256) {
+      const int64_t x = c0 + u + threadIdx.x;
+      bool pred = false;
+      if (x < ne) {
+        const unsigned long long k = D.keys[e0 + x];
+        if (k != kEmptyKey && D.vals[e0 + x] >= 0) {
+          if (mode == 0) {
+            pred = true;
+          } else {
+            int32_t kb[3];
+            unpack_key(k, kb);
+            float c[3];
+            block_center(kb, m.BL, c);
+            pred = dist2(c, s) <= m.pubD2;
+          }
         }
       }
+      const int at = wave_append(&ln, pred);
+      if (pred) lsel[at] = (uint32_t)(e0 + x);
     }
-    const int slot = wave_append(&D.ctr->sel_n, pred);
-    if (pred) D.sel[slot] = (uint32_t)(e0 + x);
+    __syncthreads();
+    if (threadIdx.x == 0) lbase = ln ? atomicAdd(&D.ctr->sel_n, ln) : 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < ln; i += blockDim.x) D.sel[lbase + i] = lsel[i];
+    __syncthreads();
   }
 }
 
@@ -1316,35 +1332,57 @@ __global__ void __launch_bounds__(256) k_export(Dev D, int map_slot, int32_t* ke
 // ---- garbage collection -----------------------------------------------------------------
 // mode 0: remove blocks beyond the gc radius (H:836-868); mode 1: remove all
 // (map teardown).
+// Removes the map's blocks beyond the GC radius (mode 0) or all of them
+// (mode 1).  Each workgroup takes kSelChunk consecutive entries and pushes
+// its freed heap blocks with one atomic per counter.
 __global__ void __launch_bounds__(256) k_gc_mark(Dev D, int map_slot, float sx, float sy, float sz, int mode) {
+  __shared__ int32_t lfree[kSelChunk];
+  __shared__ int ln, ltop, lfb;
   const MapDev m = D.maps[map_slot];
   const int64_t e0 = (int64_t)m.bucket_base * kBucket;
   const int64_t ne = (int64_t)(m.n_home + m.n_ovf) * kBucket;
   const float s[3] = {sx, sy, sz};
   unsigned long long removed = 0;
-  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < ne; x += (int64_t)gridDim.x * blockDim.x) {
-    const unsigned long long k = D.keys[e0 + x];
-    if (k == kEmptyKey) continue;
-    bool far = true;
-    if (mode == 0) {
-      int32_t kb[3];
-      unpack_key(k, kb);
-      float c[3];
-      block_center(kb, m.BL, c);
-      far = !(dist2(c, s) <= m.gcD2);
+  for (int64_t c0 = (int64_t)blockIdx.x * kSelChunk; c0 < ne; c0 += (int64_t)gridDim.x * kSelChunk) {
+    if (threadIdx.x == 0) ln = 0;
+    __syncthreads();
+    for (int u = 0; u < kSelChunk; u += 256) {
+      const int64_t x = c0 + u + threadIdx.x;
+      int slot = -1;
+      if (x < ne) {
+        const unsigned long long k = D.keys[e0 + x];
+        if (k != kEmptyKey) {
+          bool drop = true;
+          if (mode == 0) {
+            int32_t kb[3];
+            unpack_key(k, kb);
+            float c[3];
+            block_center(kb, m.BL, c);
+            drop = !(dist2(c, s) <= m.gcD2);
+          }
+          if (drop) {
+            slot = D.vals[e0 + x];
+            D.keys[e0 + x] = kEmptyKey;
+            D.vals[e0 + x] = -1;
+            removed++;
+          }
+        }
+      }
+      const int at = wave_append(&ln, slot >= 0);
+      if (slot >= 0) lfree[at] = slot;
     }
-    if (!far) continue;
-    const int slot = D.vals[e0 + x];
-    D.keys[e0 + x] = kEmptyKey;
-    D.vals[e0 + x] = -1;
-    if (slot >= 0) {
-      const int idx = atomicAdd(&D.ctr->heap_top, 1);
-      D.heap_free[idx] = slot;
-      const int f = atomicAdd(&D.ctr->freed_n, 1);
-      D.freed[f] = slot;
-      atomicSub(&D.mctr[map_slot].blocks, 1);
+    __syncthreads();
+    if (threadIdx.x == 0 && ln) {
+      ltop = atomicAdd(&D.ctr->heap_top, ln);
+      lfb = atomicAdd(&D.ctr->freed_n, ln);
+      atomicSub(&D.mctr[map_slot].blocks, ln);
     }
-    removed++;
+    __syncthreads();
+    for (int i = threadIdx.x; i < ln; i += blockDim.x) {
+      D.heap_free[ltop + i] = lfree[i];
+      D.freed[lfb + i] = lfree[i];
+    }
+    __syncthreads();
   }
   stat_add(D, ST_GC_REMOVED, removed);
 }

@@ -511,7 +511,7 @@ int publish_impl(tsdf_map* m, const float sensor[3], float* pos, float* data, in
   CHK(timing_mark(E, tid, 0));
   hipLaunchKernelGGL(k_begin_select, dim3(1), dim3(1), 0, st, E->ctr, (int)ST_PUBLISHED);
   const int64_t ne = (int64_t)(m->desc.n_home + m->desc.n_ovf) * kBucket;
-  hipLaunchKernelGGL(k_select, dim3(grid_for(ne, 256, 4096)), dim3(256), 0, st, D, m->slot, sensor[0], sensor[1], sensor[2], 1);
+  hipLaunchKernelGGL(k_select, dim3(grid_for(ne, kSelChunk, 4096)), dim3(256), 0, st, D, m->slot, sensor[0], sensor[1], sensor[2], 1);
   const int gq = grid_for(ne, 4, 2048);
   hipLaunchKernelGGL(k_pub_count<State>, dim3(gq), dim3(256), 0, st, D);
   CountPlain cp{E->pub_cnt};
@@ -566,7 +566,7 @@ int gc_impl(tsdf_map* m, const float sensor[3], int mode, int64_t* removed) {
   hipLaunchKernelGGL(k_begin_select, dim3(1), dim3(1), 0, st, E->ctr, (int)ST_GC_REMOVED);
   hipLaunchKernelGGL(k_stats_zero, dim3(4), dim3(256), 0, st, D, 1u << ST_GC_REMOVED);
   const int64_t ne = (int64_t)(m->desc.n_home + m->desc.n_ovf) * kBucket;
-  hipLaunchKernelGGL(k_gc_mark, dim3(grid_for(ne, 256, 4096)), dim3(256), 0, st, D, m->slot, sensor[0], sensor[1],
+  hipLaunchKernelGGL(k_gc_mark, dim3(grid_for(ne, kSelChunk, 4096)), dim3(256), 0, st, D, m->slot, sensor[0], sensor[1],
                      sensor[2], mode);
   hipLaunchKernelGGL(k_gc_zero<State>, dim3(1024), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k_gc_chains, dim3(grid_for(m->desc.n_home, 256, 4096)), dim3(256), 0, st, D, m->slot);
@@ -864,7 +864,7 @@ int tsdf_map_export(tsdf_map* m, int32_t* keys, float* voxels, int64_t capacity,
   hipStream_t st = E->stream;
   hipLaunchKernelGGL(k_begin_select, dim3(1), dim3(1), 0, st, E->ctr, -1);
   const int64_t ne = (int64_t)(m->desc.n_home + m->desc.n_ovf) * kBucket;
-  hipLaunchKernelGGL(k_select, dim3(grid_for(ne, 256, 4096)), dim3(256), 0, st, D, m->slot, 0.f, 0.f, 0.f, 0);
+  hipLaunchKernelGGL(k_select, dim3(grid_for(ne, kSelChunk, 4096)), dim3(256), 0, st, D, m->slot, 0.f, 0.f, 0.f, 0);
   HIPCHK(hipGetLastError());
   CHK(read_ctr(E));
   const int64_t n = E->ctr_host->sel_n;
