@@ -6,7 +6,7 @@ highest throughput. The sample is a list of batches, each a list of scans of ind
 Knobs:
   traverse_block  threads per workgroup of the binning kernel (64, 128, 256): one ray per lane, so this is the
                   number of rays whose blocks a workgroup gathers in its LDS table before touching the hash
-  fold_grid       workgroups of the first fold launch
+  fold_grid       workgroups of the first fold launch (work items are spread over them)
   buckets_log2    home buckets per map; a bucket holds 8 entries, so 17..21 spans 2^20..2^24 entries
   max_chain       longest overflow chain a map may grow (overflow-list length)
 
@@ -26,11 +26,11 @@ from .engine import BatchGraph, Engine
 
 SPACE = {
     "traverse_block": (64, 128, 256),
-    "fold_grid": (512, 1024, 2048, 4096),
+    "fold_grid": (2048, 4096, 8192, 16384),
     "buckets_log2": (17, 18, 19, 20, 21),
     "max_chain": (4, 16, 64),
 }
-DEFAULT = {"traverse_block": 256, "fold_grid": 2048, "buckets_log2": 17, "max_chain": 64}
+DEFAULT = {"traverse_block": 256, "fold_grid": 4096, "buckets_log2": 17, "max_chain": 64}
 
 
 def measure(engine: Engine, params, nmaps: int, batches, points, cfg: dict, reps: int = 1) -> dict:

@@ -55,8 +55,7 @@ struct EngCtr {
   int32_t sel_n;      // selected blocks (publish / export)
   int32_t freed_n;    // blocks freed by the current gc
   int32_t pad[3];
-  int32_t fold_n[6];  // touched blocks per fold size class
-  int32_t pad2[2];
+  int32_t fold_n[8];  // touched blocks per fold size class
   unsigned long long stats[ST_COUNT];
 };
 
@@ -105,7 +104,7 @@ struct Dev {
   int4* fold_desc;           // per size class c, from fold_base[c]: {record base, records, heap block, map}
   unsigned long long* part;  // [ST_COUNT][kPartSlots] statistics partial sums
   int64_t fold_cap;
-  int64_t fold_base[6];      // first descriptor of each size class
+  int64_t fold_base[8];      // first descriptor of each size class
   unsigned long long* seg;   // [records] update records, contiguous per touched block
   int64_t seg_cap;
   uint32_t* sel;             // [entries]

@@ -42,20 +42,21 @@ constexpr int kScanTile = 1024;  // elements per workgroup in the scan kernels
 
 // Size classes of touched blocks (records per block): the fold kernel and
 // the number of blocks folded together differ per class.
-enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WG, FC_WG2, FC_HUGE, FC_COUNT };
+enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WGL, FC_WG, FC_WG2, FC_HUGE, FC_COUNT };
+constexpr int kWgSplit = 256;  // FC_WGL: 65..256 records, FC_WG: 257..kCountTileS
 static_assert(FC_COUNT <= (int)(sizeof(EngCtr::fold_n) / sizeof(int32_t)), "fold class counters");
 
 // Descriptors a class can need when a pass holds at most `blocks` touched
 // blocks and `records` records: every block of class c holds at least lo
 // records (the class's lower size bound).
 __host__ __device__ constexpr int64_t fold_class_cap(int c, int64_t blocks, int64_t records) {
-  const int64_t lo = c == FC_16 ? 1 : c == FC_32 ? 17 : c == FC_64 ? 33 : c == FC_WG ? 65 : c == FC_WG2 ? kCountTileS + 1 : kCountTile + 1;
+  const int64_t lo = c == FC_16 ? 1 : c == FC_32 ? 17 : c == FC_64 ? 33 : c == FC_WGL ? 65 : c == FC_WG ? kWgSplit + 1 : c == FC_WG2 ? kCountTileS + 1 : kCountTile + 1;
   const int64_t r = records / lo + 1;
   return r < blocks ? r : blocks;
 }
 
 __host__ __device__ constexpr int fold_class(int n) {
-  return n <= 16 ? FC_16 : n <= 32 ? FC_32 : n <= 64 ? FC_64 : n <= kCountTileS ? FC_WG : n <= kCountTile ? FC_WG2 : FC_HUGE;
+  return n <= 16 ? FC_16 : n <= 32 ? FC_32 : n <= 64 ? FC_64 : n <= kWgSplit ? FC_WGL : n <= kCountTileS ? FC_WG : n <= kCountTile ? FC_WG2 : FC_HUGE;
 }
 
 // ---- wave / workgroup helpers -------------------------------------------------
@@ -901,18 +902,17 @@ __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
 // holds one block's records, bitonic-sorts them with shuffles, and every
 // run of one voxel is folded by its first lane pulling the run's weight
 // terms from the following lanes.
+// One wave folds the class-G descriptors [q_begin, q_end), 64/G at a time.
 template <class State, int G>
-__device__ __forceinline__ void fold_small_class(const Dev& D, unsigned long long& uniq) {
+__device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int q_end, unsigned long long& uniq) {
   constexpr int kPer = 64 / G;
   constexpr int kCls = G == 16 ? FC_16 : G == 32 ? FC_32 : FC_64;
   const int lane = lane_id();
   const int gl = lane & (G - 1);
   const int grp = lane / G;
-  const int n_desc = ld_acq(&D.ctr->fold_n[kCls]);
+  const int n_desc = q_end;
   const int4* desc = D.fold_desc + D.fold_base[kCls];
-  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int nw = gridDim.x * (blockDim.x >> 6);
-  for (int q0 = gw * kPer; q0 < n_desc; q0 += nw * kPer) {
+  for (int q0 = q_begin; q0 < n_desc; q0 += kPer) {
     const int q = q0 + grp;
     int4 d = make_int4(0, 0, 0, 0);
     if (q < n_desc) d = desc[q];
@@ -1159,9 +1159,12 @@ __device__ __forceinline__ void fold_huge_block(const Dev& D, const int4 d, unsi
   __syncthreads();
 }
 
-// Fold, first launch: blocks of 65..kCountTileS records (one workgroup
-// each, 20 KB of LDS, so up to eight workgroups share a CU), then the small
-// classes (several blocks per wave).
+// Fold, first launch.  Work items, in this order: the blocks of 257..1024
+// records, those of 65..256 (a workgroup each: 20 KB of LDS, so up to eight
+// workgroups share a CU), then chunks of the small classes (one group of
+// 64/G blocks per wave).  Workgroup q takes items q, q + grid, ...; with a
+// grid larger than what fits on the chip at once, the dispatcher starts a
+// new workgroup whenever one finishes, which balances the uneven items.
 template <class State>
 __global__ void __launch_bounds__(256) k_fold_a(Dev D) {
   __shared__ unsigned long long rin[kCountTileS];
@@ -1171,12 +1174,37 @@ __global__ void __launch_bounds__(256) k_fold_a(Dev D) {
   __shared__ uint32_t sh[16];
   unsigned long long uniq = 0;
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG]);
-  const int4* desc = D.fold_desc + D.fold_base[FC_WG];
-  for (int q = blockIdx.x; q < n_wg; q += gridDim.x)
-    fold_count_block<State, kCountTileS>(D, desc[q], rin, rout, cnt, start, sh, uniq);
-  fold_small_class<State, 64>(D, uniq);
-  fold_small_class<State, 32>(D, uniq);
-  fold_small_class<State, 16>(D, uniq);
+  const int n_wgl = ld_acq(&D.ctr->fold_n[FC_WGL]);
+  const int n64 = ld_acq(&D.ctr->fold_n[FC_64]);
+  const int n32 = ld_acq(&D.ctr->fold_n[FC_32]);
+  const int n16 = ld_acq(&D.ctr->fold_n[FC_16]);
+  const int waves = blockDim.x >> 6, wave = threadIdx.x >> 6;
+  // blocks per small-class item: one group of 64/G blocks per wave
+  const int k64 = waves * 1, k32 = waves * 2, k16 = waves * 4;
+  const int c64 = (n64 + k64 - 1) / k64, c32 = (n32 + k32 - 1) / k32, c16 = (n16 + k16 - 1) / k16;
+  const int total = n_wg + n_wgl + c64 + c32 + c16;
+  const int4* dwg = D.fold_desc + D.fold_base[FC_WG];
+  const int4* dwgl = D.fold_desc + D.fold_base[FC_WGL];
+  for (int it = blockIdx.x; it < total; it += gridDim.x) {
+    if (it < n_wg) {
+      fold_count_block<State, kCountTileS>(D, dwg[it], rin, rout, cnt, start, sh, uniq);
+    } else if (it < n_wg + n_wgl) {
+      fold_count_block<State, kCountTileS>(D, dwgl[it - n_wg], rin, rout, cnt, start, sh, uniq);
+    } else {
+      int c = it - n_wg - n_wgl;
+      if (c < c64) {
+        const int q = c * k64 + wave;
+        fold_small_range<State, 64>(D, q, min(q + 1, n64), uniq);
+      } else if ((c -= c64) < c32) {
+        const int q = c * k32 + wave * 2;
+        fold_small_range<State, 32>(D, q, min(q + 2, n32), uniq);
+      } else {
+        c -= c32;
+        const int q = c * k16 + wave * 4;
+        fold_small_range<State, 16>(D, q, min(q + 4, n16), uniq);
+      }
+    }
+  }
   stat_add(D, ST_UNIQUE, uniq);
 }
 

@@ -141,7 +141,7 @@ struct tsdf_engine {
   int4* fold_desc = nullptr;
   unsigned long long* part = nullptr;
   int64_t fold_cap = 0;
-  int64_t fold_base[6] = {};
+  int64_t fold_base[8] = {};
   uint8_t* touched_map = nullptr;
   uint32_t* sel = nullptr;
   uint32_t* pub_cnt = nullptr;
@@ -161,7 +161,7 @@ struct tsdf_engine {
   int64_t last_points = 0;
 
   // launch knobs
-  tsdf_launch_config launch{256, 65536, 2048, 0};
+  tsdf_launch_config launch{256, 65536, 4096, 0};
 
   // timing
   bool timing = false;
@@ -202,7 +202,7 @@ struct tsdf_engine {
     D.fold_desc = fold_desc;
     D.part = part;
     D.fold_cap = fold_cap;
-    for (int c = 0; c < 6; ++c) D.fold_base[c] = fold_base[c];
+    for (int c = 0; c < 8; ++c) D.fold_base[c] = fold_base[c];
     D.touched_map = touched_map;
     D.seg = seg;
     D.seg_cap = seg_cap;

@@ -102,7 +102,7 @@ class Engine:
         mp = (C.c_void_p * len(maps))(*[m.h for m in maps])
         check(abi.load().tsdf_integrate_batch(self.h, mp, len(maps), arr, len(scans), C.c_void_p(points.ptr)))
 
-    def set_launch(self, traverse_block=256, traverse_grid_cap=65536, fold_grid=2048):
+    def set_launch(self, traverse_block=256, traverse_grid_cap=65536, fold_grid=4096):
         cfg = abi.LaunchConfig(traverse_block, traverse_grid_cap, fold_grid, 0)
         check(abi.load().tsdf_engine_set_launch(self.h, C.byref(cfg)))
 
