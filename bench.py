@@ -142,10 +142,28 @@ def main():
             traffic = tj.get(dom_kernel)
         except Exception:
             traffic = None
+    # the same kernel against the VALU issue rate: its instruction count per launch (rocprofv3
+    # SQ_INSTS_VALU, profiles/valu_dense.json) over its live duration
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", "valu_dense.json")
+    if os.path.exists(vpath):
+        try:
+            with open(vpath) as f:
+                vj = json.load(f)
+            members = [k.split("<")[0] if k.startswith("k_scan_tiles") else k for k in dom_kernel.split("+")]
+            members = [next((n for n in vj if n.split("<")[0] == m), m) for m in members]
+            if all(m in vj for m in members):
+                n_valu = sum(vj[m]["valu"] for m in members)
+                rate = n_valu / (stage_ms[dom] * 1e-3)
+                peak = vj["_peak_wave_insts_per_s"]
+                valu = {"unit": "G wave64 VALU instructions/s", "instructions_per_launch": n_valu,
+                        "achieved": rate / 1e9, "peak": peak / 1e9, "frac": rate / peak}
+        except Exception:
+            valu = None
     roofline = {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                 "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
                 "alg_bytes_per_launch": alg_bytes, "bytes_per_point": float(np.mean(bpt)),
-                "pass_achieved_GBps": alg_bytes / (stage_ms["total"] * 1e-3) / 1e9}
+                "pass_achieved_GBps": alg_bytes / (stage_ms["total"] * 1e-3) / 1e9, "valu": valu}
     log("plain: %.3e pts/s, %.3f ms/scan, stages %s" % (value, ms_per_step, {k: round(v, 4) for k, v in stage_ms.items()}))
     result["stages_ms"] = {k: round(v, 5) for k, v in stage_ms.items()}
     result["per_scan"] = {"points": float(np.mean(N)), "voxel_updates": float(np.mean(M)),
