@@ -138,8 +138,8 @@ def load():
         return _lib
     path = library_path()
     if not os.path.exists(path):
-        raise RuntimeError("HIP engine library %s is missing: run __graft_entry__.build() "
-                           "(or python -m ros_tsdf_amd._build) first" % path)
+        raise RuntimeError("HIP engine library %s is missing: build it first "
+                           "(python -m ros_tsdf_amd._build)" % path)
     L = C.CDLL(path)
     for name, args in SIGNATURES.items():
         f = getattr(L, name)
