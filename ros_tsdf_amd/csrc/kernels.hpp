@@ -362,8 +362,17 @@ __device__ __forceinline__ int run_step(uint32_t code, int axis) {
   return v == 3 ? -1 : v;
 }
 
+// Slot of an LDS-table key: the 48 coordinate bits and the map folded into
+// 32 bits and mixed with one multiply (the table only spreads the few
+// hundred blocks of one workgroup, so a light hash suffices).
+__device__ __forceinline__ uint32_t lds_hash(unsigned long long k) {
+  const uint32_t x = (uint32_t)k ^ (uint32_t)(k >> 29) ^ (uint32_t)(k >> 48) * 0x9E3779B1u;
+  return (x * 0x85EBCA6Bu) >> (32 - 10);
+}
+static_assert(kBinSlots == 1024, "lds_hash yields 10 bits");
+
 __device__ __forceinline__ int lds_find_or_add(unsigned long long* tkey, unsigned long long k) {
-  uint32_t h = hash_key(k) & (kBinSlots - 1);
+  uint32_t h = lds_hash(k);
   for (int probe = 0; probe < 64; ++probe) {
     const unsigned long long cur = tkey[h];
     if (cur == k) return (int)h;
@@ -762,10 +771,6 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
     int j = -1;
     int64_t at = -1;
     bool tv;
-    // cell centres change on one or two axes per step: each axis's centre is
-    // recomputed only when its index moves
-    int32_t kc[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-    float vc[3] = {0.f, 0.f, 0.f};
     walk_cells(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
       const unsigned long long key = pack_key(block_of(k0), block_of(k1), block_of(k2));
       if (key != rkey) {
@@ -778,12 +783,10 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
       }
       if (at >= 0) {
         const int32_t kk[3] = {k0, k1, k2};
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax)
-          if (kk[ax] != kc[ax]) {
-            kc[ax] = kk[ax];
-            vc[ax] = cell_centre_of(kk[ax], m.sc_v, m.isc_v, m.half_vs);
-          }
+        // centres from the integer cell (a product and an FMA per axis; no
+        // branches, which cost more than recomputing under divergence)
+        const float vc[3] = {cell_centre_of(k0, m.sc_v, m.isc_v, m.half_vs), cell_centre_of(k1, m.sc_v, m.isc_v, m.half_vs),
+                             cell_centre_of(k2, m.sc_v, m.isc_v, m.half_vs)};
         D.seg[at++] = make_rec(local_index(kk), i, signed_distance(p, s, vc));
       }
       return true;

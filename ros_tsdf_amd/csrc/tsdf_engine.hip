@@ -402,7 +402,9 @@ int enqueue_pass(tsdf_engine* E, int64_t P, int tid) {
   // (heap assignment runs inside k_bin: the assign stage is empty)
   CHK(timing_mark(E, tid, 2));
   const int64_t tmax = std::min<int64_t>(E->entries, P * E->stride_cap);
-  const int sg = grid_for(tmax, kScanTile, 1024);
+  // tiles of touched blocks are taken grid-stride by at most 256 workgroups
+  // (the touched count is only known on the device; most passes have tens)
+  const int sg = grid_for(tmax, kScanTile, 256);
   CountTouchedRecords ct{E->touched, E->blk_cnt};
   hipLaunchKernelGGL(k_scan_tiles<CountTouchedRecords>, dim3(sg), dim3(256), 0, st, &E->ctr->touched_n, ct, E->tile_sums);
   hipLaunchKernelGGL(k_seg_apply, dim3(sg), dim3(256), 0, st, D);
