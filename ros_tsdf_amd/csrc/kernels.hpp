@@ -667,6 +667,34 @@ __device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* sh, ui
   return r;
 }
 
+__device__ __forceinline__ unsigned long long block_scan_excl64(unsigned long long v, unsigned long long* sh,
+                                                                 unsigned long long& total) {
+  const int lane = lane_id();
+  const int w = threadIdx.x >> 6;
+  unsigned long long x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long acc = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+      const unsigned long long t = sh[k];
+      sh[k] = acc;
+      acc += t;
+    }
+    sh[15] = acc;
+  }
+  __syncthreads();
+  total = sh[15];
+  const unsigned long long r = x - v + sh[w];
+  __syncthreads();
+  return r;
+}
+
 template <class Count>
 __global__ void __launch_bounds__(256) k_scan_tiles(const int32_t* n_ptr, Count cnt, uint32_t* tile_sums) {
   __shared__ uint32_t sh[16];
@@ -838,32 +866,42 @@ __device__ __forceinline__ int rec_voxel(unsigned long long r) { return (int)(r 
 // per class and workgroup.  The pass counters of the blocks are reset for
 // the next pass, and entries that got no heap block leave the table.
 __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
-  __shared__ uint32_t sh[16];
+  __shared__ unsigned long long sh64[16];
   __shared__ int lcnt[FC_COUNT], gbase[FC_COUNT];
   const int n = ld_acq(&D.ctr->touched_n);
   const int ntiles = (n + kScanTile - 1) / kScanTile;
   unsigned long long dropped = 0;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    uint32_t c = 0;
-    for (int k = threadIdx.x; k < t; k += blockDim.x) c += D.tile_sums[k];
-    uint32_t carry;
-    block_scan_excl(c, sh, carry);
-    if (threadIdx.x < FC_COUNT) lcnt[threadIdx.x] = 0;
+    // the tile's loads first (touched entries, then their counters, heap
+    // blocks and maps, all independent of each other), the preceding tile
+    // totals beside them
     const int i0 = t * kScanTile + threadIdx.x * 4;
     uint32_t e4[4], cn[4];
-    uint32_t v = 0;
+    int sl4[4];
+    uint8_t mp4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) e4[k] = i0 + k < n ? D.touched[i0 + k] : 0u;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      e4[k] = 0;
       cn[k] = 0;
+      sl4[k] = -1;
+      mp4[k] = 0;
       if (i0 + k < n) {
-        e4[k] = D.touched[i0 + k];
         cn[k] = (uint32_t)(D.blk_cnt[e4[k]] >> 32);
+        sl4[k] = D.vals[e4[k]];
+        mp4[k] = D.touched_map[i0 + k];
       }
-      v += cn[k];
     }
-    uint32_t tot;
-    uint32_t ex = block_scan_excl(v, sh, tot) + carry;
+    uint32_t c = 0;
+    for (int k = threadIdx.x; k < t; k += blockDim.x) c += D.tile_sums[k];
+    if (threadIdx.x < FC_COUNT) lcnt[threadIdx.x] = 0;
+    const uint32_t v = cn[0] + cn[1] + cn[2] + cn[3];
+    // one scan yields both: preceding tiles' total in the high half, this
+    // thread's record offset in the low half (a tile's records fit in 32 bits)
+    unsigned long long tot;
+    const unsigned long long exv = block_scan_excl64(((unsigned long long)c << 32) | v, sh64, tot);
+    const uint32_t carry = (uint32_t)(tot >> 32);
+    uint32_t ex = (uint32_t)exv + carry;
     int cls[4], rank[4];
     int4 dsc[4];
 #pragma unroll
@@ -873,19 +911,18 @@ __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
       dsc[k] = make_int4(0, 0, 0, 0);
       if (i0 + k < n) {
         D.seg_off[i0 + k] = ex;
-        const int slot = D.vals[e4[k]];
         D.blk_cnt[e4[k]] = 0;
-        if (slot < 0) {
+        if (sl4[k] < 0) {
           dropped += cn[k];
         } else {
-          dsc[k] = make_int4((int)ex, (int)cn[k], slot, (int)D.touched_map[i0 + k]);
+          dsc[k] = make_int4((int)ex, (int)cn[k], sl4[k], (int)mp4[k]);
           cls[k] = fold_class((int)cn[k]);
           rank[k] = atomicAdd(&lcnt[cls[k]], 1);
         }
       }
       ex += cn[k];
     }
-    if (t == ntiles - 1 && threadIdx.x == 0) D.seg_off[n] = carry + tot;
+    if (t == ntiles - 1 && threadIdx.x == 0) D.seg_off[n] = carry + (uint32_t)tot;
     __syncthreads();
     if (threadIdx.x < FC_COUNT)
       gbase[threadIdx.x] = lcnt[threadIdx.x] ? atomicAdd(&D.ctr->fold_n[threadIdx.x], lcnt[threadIdx.x]) : 0;
