@@ -274,3 +274,26 @@ def test_scan_larger_than_initial_scratch(oracle_lib):
         compare_maps(m, om, D[1], *tol(D))
         m.close()
         om.close()
+
+
+@pytest.mark.parametrize("fmt", [abi.TSDF_STATE_F16, abi.TSDF_STATE_F16_QW], ids=["fp16", "fp16_fixed_weight"])
+def test_packed_state_publish_and_gc_sequence(oracle_lib, fmt):
+    """The reference call sequence (integrate -> publish -> gc, H:1003-1007) with packed voxel state: the
+    published and removed sets are exact, the values within the packed formats' tolerance (1e-2 T)."""
+    from ros_tsdf_amd import Engine
+    with Engine(heap_blocks=1 << 15, total_buckets=1 << 16, max_batch_points=1 << 14, state_format=fmt) as e:
+        m = e.map(D, buckets_log2=12)
+        om = oracle_lib.OracleMap(D)
+        for k in range(3):
+            pts, s = dense_scan(k, n=4096)
+            pos, data, n = m.process_scan(pts, s)
+            om.integrate(pts, s)
+            opos, osdf, ow = om.publish(s)
+            assert n == opos.shape[0]
+            compare_publish(pos, data, opos, osdf, ow, D[1], 1e-2, 2e-2 * D[2])
+            om.gc(s)
+        k_g, _, w_g = m.export()
+        k_o, _, w_o = om.export()
+        assert np.array_equal(k_g, k_o) and np.array_equal(w_g != 0, w_o != 0)
+        m.close()
+        om.close()
