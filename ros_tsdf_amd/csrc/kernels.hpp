@@ -371,18 +371,56 @@ __device__ __forceinline__ uint32_t lds_hash(unsigned long long k) {
 }
 static_assert(kBinSlots == 1024, "lds_hash yields 10 bits");
 
-__device__ __forceinline__ int lds_find_or_add(unsigned long long* tkey, unsigned long long k) {
+// Slot of key k in the workgroup's table, inserting it when absent (-1 when
+// the table is full).  *added is set when this call inserted it.
+__device__ __forceinline__ int lds_find_or_add(unsigned long long* tkey, unsigned long long k, bool* added = nullptr) {
   uint32_t h = lds_hash(k);
   for (int probe = 0; probe < 64; ++probe) {
     const unsigned long long cur = tkey[h];
     if (cur == k) return (int)h;
     if (cur == kEmptyKey) {
       const unsigned long long old = atomicCAS(&tkey[h], kEmptyKey, k);
+      if (old == kEmptyKey && added) *added = true;
       if (old == kEmptyKey || old == k) return (int)h;
     }
     h = (h + 1) & (kBinSlots - 1);
   }
   return -1;
+}
+
+// The packed map key and map slot of an LDS-table key.
+__device__ __forceinline__ unsigned long long table_map_key(unsigned long long k, int& map_slot) {
+  map_slot = (int)(k >> 48);
+  const int32_t k0 = (int32_t)(int16_t)((k >> 32) & 0xFFFF), k1 = (int32_t)(int16_t)((k >> 16) & 0xFFFF),
+                k2 = (int32_t)(int16_t)(k & 0xFFFF);
+  return pack_key(k0, k1, k2);
+}
+
+// The home bucket of a key, read ahead of its lookup: its index and keys.
+struct HomePeek {
+  int g;
+  ulonglong2 v0, v1, v2, v3;
+};
+__device__ __forceinline__ HomePeek peek_home(const Dev& D, const MapDev& m, unsigned long long key) {
+  HomePeek h;
+  h.g = m.bucket_base + (int)(hash_key(key) & m.home_mask);
+  const ulonglong2* bk = reinterpret_cast<const ulonglong2*>(D.keys + (int64_t)h.g * kBucket);
+  h.v0 = bk[0];
+  h.v1 = bk[1];
+  h.v2 = bk[2];
+  h.v3 = bk[3];
+  return h;
+}
+// Entry of key when the peeked bucket holds it, else -1.  Slots only go
+// from empty to full while binning runs, so a key seen in an earlier read
+// of the bucket is still there; one not seen takes the full lookup.
+__device__ __forceinline__ int peek_find(const HomePeek& h, unsigned long long key) {
+  const unsigned long long kk[kBucket] = {h.v0.x, h.v0.y, h.v1.x, h.v1.y, h.v2.x, h.v2.y, h.v3.x, h.v3.y};
+  int e = -1;
+#pragma unroll
+  for (int s = kBucket - 1; s >= 0; --s)
+    if (kk[s] == key) e = h.g * kBucket + s;
+  return e;
 }
 
 __global__ void __launch_bounds__(256) k_bin(Dev D) {
@@ -419,6 +457,32 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
       count_assign(assign_heap_one(D, (uint32_t)e, ms));
     }
   };
+  // global lookup of table slot x (phase B)
+  auto resolve_slot = [&](int x, int e) {
+    const unsigned long long k = tkey[x];
+    int ms;
+    const unsigned long long key = table_map_key(k, ms);
+    if (e < 0) {
+      bool ins;
+      e = find_or_insert(D, D.maps[ms], key, ins);
+      if (e >= 0 && ins) note_new(e, ms);
+    }
+    tent[x] = e;
+    if (e < 0) {
+      st[6]++;
+      return;
+    }
+    const uint32_t nrec = trec[x];
+    if (nrec > 0) {
+      const unsigned long long old = atomicAdd(&D.blk_cnt[e], (unsigned long long)nrec << 32);
+      tbase[x] = (uint32_t)(old >> 32);
+      if (old == 0) {
+        const int at = atomicAdd(&n_touch, 1);
+        ltouch[at] = (uint32_t)e;
+        ltouch_map[at] = (uint8_t)ms;
+      }
+    }
+  };
   const int64_t stride = D.pcap;
   for (int base = blockIdx.x * blockDim.x; base < P; base += gridDim.x * blockDim.x) {
     for (int x = threadIdx.x; x < kBinSlots; x += blockDim.x) {
@@ -431,20 +495,28 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
       n_occ = 0;
     }
     __syncthreads();
-    // ---- phase A: walks; blocks gathered in the LDS table ----
+    // ---- phase A1: block walks; blocks gathered in the LDS table ----
     const int i = base + threadIdx.x;
     int nj = 0;
+    bool walk = false;
+    ScanDev sd;
+    MapDev m;
+    float p[3], s[3], a[3], b[3];
     if (i < P) {
-      const ScanDev sd = D.scans[find_scan(D.scans, hdr.nscans, i)];
-      const MapDev m = D.maps[sd.map];
+      sd = D.scans[find_scan(D.scans, hdr.nscans, i)];
+      m = D.maps[sd.map];
       const float4 q = hdr.points[i];
-      const float p[3] = {q.x, q.y, q.z};
-      const float s[3] = {sd.s[0], sd.s[1], sd.s[2]};
+      p[0] = q.x;
+      p[1] = q.y;
+      p[2] = q.z;
+      s[0] = sd.s[0];
+      s[1] = sd.s[1];
+      s[2] = sd.s[2];
       if (!point_valid(p, s, m.vs)) {
         st[1]++;
       } else {
         st[0]++;
-        float a[3], b[3];
+        walk = true;
         truncation_segment(p, s, m.T, a, b);
         bool tb = false;
         const int nb = walk_cells(a, b, m.BL, m.sc_b, m.isc_b, m.half_BL, m.max_blk, tb, [&](int32_t k0, int32_t k1, int32_t k2) {
@@ -459,97 +531,11 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
         });
         st[2] += nb;
         st[5] += tb ? 1 : 0;
-        // A run (the voxels of one block, consecutive along the walk) gets
-        // its record offset in the block's LDS slot, or, past a full table,
-        // straight from the global counter.
-        auto close_run = [&](const int32_t kb[3], int rn) {
-          const int slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb));
-          if (slot >= 0) {
-            const uint32_t local = atomicAdd(&trec[slot], (uint32_t)rn);
-            D.pair_e[(int64_t)nj * stride + i] = (uint32_t)slot;
-            D.pair_r[(int64_t)nj * stride + i] = local | ((uint32_t)rn << 24);
-          } else {
-            bool ins;
-            const int e = find_or_insert(D, m, pack_key(kb[0], kb[1], kb[2]), ins);
-            if (e < 0) {
-              st[6]++;
-              st[7] += rn;
-              D.pair_e[(int64_t)nj * stride + i] = kPairFail;
-            } else {
-              if (ins) note_new(e, m.slot);
-              const unsigned long long old = atomicAdd(&D.blk_cnt[e], (unsigned long long)rn << 32);
-              if (old == 0) {
-                const int t = atomicAdd(&D.ctr->touched_n, 1);
-                D.touched[t] = (uint32_t)e;
-                D.touched_map[t] = (uint8_t)m.slot;
-                D.touch_idx[e] = (uint32_t)t;
-              }
-              D.pair_e[(int64_t)nj * stride + i] = kPairDirect | (uint32_t)e;
-              D.pair_r[(int64_t)nj * stride + i] = (uint32_t)(old >> 32);
-              st[3] += rn;
-            }
-          }
-          ++nj;
-        };
-        // The voxel walk only records its runs, in two registers: 6 bits of
-        // length per run, and 6 bits of block step from the previous run (2
-        // bits per axis: consecutive runs' blocks differ by at most one per
-        // axis).  The table work for the runs follows the walk, so the lanes
-        // of a wave do it together rather than at scattered steps of their
-        // walks.  A walk of more than kPackedRuns runs flushes mid-way.
-        unsigned long long lens = 0, steps = 0;
-        int nrun = 0, rn = 0;
-        int32_t fb[3] = {0, 0, 0}, pb[3] = {0, 0, 0};  // flush base block, current run's block
-        auto flush_runs = [&]() {
-          int32_t cb[3] = {fb[0], fb[1], fb[2]};
-          for (int j = 0; j < nrun; ++j) {
-            const uint32_t code = (uint32_t)(steps >> (6 * j)) & 63u;
-            cb[0] += run_step(code, 0);
-            cb[1] += run_step(code, 1);
-            cb[2] += run_step(code, 2);
-            close_run(cb, (int)((lens >> (6 * j)) & 63u));
-          }
-          nrun = 0;
-          lens = 0;
-          steps = 0;
-          fb[0] = cb[0];
-          fb[1] = cb[1];
-          fb[2] = cb[2];
-        };
-        bool tv = false;
-        walk_cells(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
-          const int32_t kb[3] = {block_of(k0), block_of(k1), block_of(k2)};
-          if (rn == 0) {
-            fb[0] = pb[0] = kb[0];
-            fb[1] = pb[1] = kb[1];
-            fb[2] = pb[2] = kb[2];
-            nrun = 1;
-            rn = 1;
-            return true;
-          }
-          if (kb[0] != pb[0] || kb[1] != pb[1] || kb[2] != pb[2]) {
-            lens |= (unsigned long long)rn << (6 * (nrun - 1));
-            if (nrun == kPackedRuns) flush_runs();
-            steps |= (unsigned long long)run_code(kb[0] - pb[0], kb[1] - pb[1], kb[2] - pb[2]) << (6 * nrun);
-            ++nrun;
-            rn = 0;
-            pb[0] = kb[0];
-            pb[1] = kb[1];
-            pb[2] = kb[2];
-          }
-          ++rn;
-          return true;
-        });
-        if (nrun > 0) {
-          lens |= (unsigned long long)rn << (6 * (nrun - 1));
-          flush_runs();
-        }
-        st[4] += tv ? 1 : 0;
       }
-      D.pair_n[i] = (uint8_t)nj;
     }
     __syncthreads();
-    // ---- phase B: one global lookup and one counter atomic per gathered block ----
+    // ---- the table's blocks listed; their home buckets are read now and
+    // used after the voxel walks, which hide the memory latency ----
     for (int x0 = 0; x0 < kBinSlots; x0 += blockDim.x) {
       const int x = x0 + threadIdx.x;
       const bool occ = x < kBinSlots && tkey[x] != kEmptyKey;
@@ -557,31 +543,119 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
       if (occ) locc[at] = (uint16_t)x;
     }
     __syncthreads();
-    for (int y = threadIdx.x; y < n_occ; y += blockDim.x) {
-      const int x = locc[y];
-      const unsigned long long k = tkey[x];
-      const int ms = (int)(k >> 48);
-      const MapDev& m = D.maps[ms];
-      const int32_t k0 = (int32_t)(int16_t)((k >> 32) & 0xFFFF), k1 = (int32_t)(int16_t)((k >> 16) & 0xFFFF),
-                    k2 = (int32_t)(int16_t)(k & 0xFFFF);
-      bool ins;
-      const int e = find_or_insert(D, m, pack_key(k0, k1, k2), ins);
-      tent[x] = e;
-      if (e < 0) {
-        st[6]++;
-        continue;
-      }
-      if (ins) note_new(e, ms);
-      const uint32_t nrec = trec[x];
-      if (nrec > 0) {
-        const unsigned long long old = atomicAdd(&D.blk_cnt[e], (unsigned long long)nrec << 32);
-        tbase[x] = (uint32_t)(old >> 32);
-        if (old == 0) {
-          const int at = atomicAdd(&n_touch, 1);
-          ltouch[at] = (uint32_t)e;
-          ltouch_map[at] = (uint8_t)ms;
+    const int n_listed = n_occ;
+    HomePeek pk0, pk1;
+    unsigned long long key0 = kEmptyKey, key1 = kEmptyKey;
+    if ((int)threadIdx.x < n_listed) {
+      int ms;
+      key0 = table_map_key(tkey[locc[threadIdx.x]], ms);
+      pk0 = peek_home(D, D.maps[ms], key0);
+    }
+    if ((int)(threadIdx.x + blockDim.x) < n_listed) {
+      int ms;
+      key1 = table_map_key(tkey[locc[threadIdx.x + blockDim.x]], ms);
+      pk1 = peek_home(D, D.maps[ms], key1);
+    }
+    // ---- phase A2: voxel walks ----
+    if (walk) {
+      // A run (the voxels of one block, consecutive along the walk) gets
+      // its record offset in the block's LDS slot, or, past a full table,
+      // straight from the global counter.  A block the block walk did not
+      // list joins the table's list here.
+      auto close_run = [&](const int32_t kb[3], int rn) {
+        bool added = false;
+        const int slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb), &added);
+        if (slot >= 0) {
+          if (added) locc[atomicAdd(&n_occ, 1)] = (uint16_t)slot;
+          const uint32_t local = atomicAdd(&trec[slot], (uint32_t)rn);
+          D.pair_e[(int64_t)nj * stride + i] = (uint32_t)slot;
+          D.pair_r[(int64_t)nj * stride + i] = local | ((uint32_t)rn << 24);
+        } else {
+          bool ins;
+          const int e = find_or_insert(D, m, pack_key(kb[0], kb[1], kb[2]), ins);
+          if (e < 0) {
+            st[6]++;
+            st[7] += rn;
+            D.pair_e[(int64_t)nj * stride + i] = kPairFail;
+          } else {
+            if (ins) note_new(e, m.slot);
+            const unsigned long long old = atomicAdd(&D.blk_cnt[e], (unsigned long long)rn << 32);
+            if (old == 0) {
+              const int t = atomicAdd(&D.ctr->touched_n, 1);
+              D.touched[t] = (uint32_t)e;
+              D.touched_map[t] = (uint8_t)m.slot;
+              D.touch_idx[e] = (uint32_t)t;
+            }
+            D.pair_e[(int64_t)nj * stride + i] = kPairDirect | (uint32_t)e;
+            D.pair_r[(int64_t)nj * stride + i] = (uint32_t)(old >> 32);
+            st[3] += rn;
+          }
         }
+        ++nj;
+      };
+      // The voxel walk only records its runs, in two registers: 6 bits of
+      // length per run, and 6 bits of block step from the previous run (2
+      // bits per axis: consecutive runs' blocks differ by at most one per
+      // axis).  The table work for the runs follows the walk, so the lanes
+      // of a wave do it together rather than at scattered steps of their
+      // walks.  A walk of more than kPackedRuns runs flushes mid-way.
+      unsigned long long lens = 0, steps = 0;
+      int nrun = 0, rn = 0;
+      int32_t fb[3] = {0, 0, 0}, pb[3] = {0, 0, 0};  // flush base block, current run's block
+      auto flush_runs = [&]() {
+        int32_t cb[3] = {fb[0], fb[1], fb[2]};
+        for (int j = 0; j < nrun; ++j) {
+          const uint32_t code = (uint32_t)(steps >> (6 * j)) & 63u;
+          cb[0] += run_step(code, 0);
+          cb[1] += run_step(code, 1);
+          cb[2] += run_step(code, 2);
+          close_run(cb, (int)((lens >> (6 * j)) & 63u));
+        }
+        nrun = 0;
+        lens = 0;
+        steps = 0;
+        fb[0] = cb[0];
+        fb[1] = cb[1];
+        fb[2] = cb[2];
+      };
+      bool tv = false;
+      walk_cells(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
+        const int32_t kb[3] = {block_of(k0), block_of(k1), block_of(k2)};
+        if (rn == 0) {
+          fb[0] = pb[0] = kb[0];
+          fb[1] = pb[1] = kb[1];
+          fb[2] = pb[2] = kb[2];
+          nrun = 1;
+          rn = 1;
+          return true;
+        }
+        if (kb[0] != pb[0] || kb[1] != pb[1] || kb[2] != pb[2]) {
+          lens |= (unsigned long long)rn << (6 * (nrun - 1));
+          if (nrun == kPackedRuns) flush_runs();
+          steps |= (unsigned long long)run_code(kb[0] - pb[0], kb[1] - pb[1], kb[2] - pb[2]) << (6 * nrun);
+          ++nrun;
+          rn = 0;
+          pb[0] = kb[0];
+          pb[1] = kb[1];
+          pb[2] = kb[2];
+        }
+        ++rn;
+        return true;
+      });
+      if (nrun > 0) {
+        lens |= (unsigned long long)rn << (6 * (nrun - 1));
+        flush_runs();
       }
+      st[4] += tv ? 1 : 0;
+    }
+    if (i < P) D.pair_n[i] = (uint8_t)nj;
+    __syncthreads();
+    // ---- phase B: one global lookup and one counter atomic per listed block
+    // (the first two per thread from the buckets read ahead) ----
+    for (int y = threadIdx.x, r = 0; y < n_occ; y += blockDim.x, ++r) {
+      int e = -1;
+      if (y < n_listed && r < 2) e = r == 0 ? peek_find(pk0, key0) : peek_find(pk1, key1);
+      resolve_slot(locc[y], e);
     }
     __syncthreads();
     if (threadIdx.x == 0) g_touch = n_touch ? atomicAdd(&D.ctr->touched_n, n_touch) : 0;
@@ -639,8 +713,11 @@ struct CountPlain {
   __device__ uint32_t operator()(int i) const { return cnt[i]; }
 };
 
+// Workgroup exclusive scans (up to 1024 threads, wave64): inclusive wave
+// scan, then a scan of the wave totals.  sh holds kScanSh entries: the
+// waves' offsets, then the total.
+constexpr int kScanSh = 17;
 __device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* sh, uint32_t& total) {
-  // 256 threads, wave64: inclusive wave scan, then scan of wave totals.
   const int lane = lane_id();
   const int w = threadIdx.x >> 6;
   uint32_t x = v;
@@ -658,10 +735,10 @@ __device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* sh, ui
       sh[k] = acc;
       acc += t;
     }
-    sh[15] = acc;
+    sh[kScanSh - 1] = acc;
   }
   __syncthreads();
-  total = sh[15];
+  total = sh[kScanSh - 1];
   const uint32_t r = x - v + sh[w];
   __syncthreads();
   return r;
@@ -686,10 +763,10 @@ __device__ __forceinline__ unsigned long long block_scan_excl64(unsigned long lo
       sh[k] = acc;
       acc += t;
     }
-    sh[15] = acc;
+    sh[kScanSh - 1] = acc;
   }
   __syncthreads();
-  total = sh[15];
+  total = sh[kScanSh - 1];
   const unsigned long long r = x - v + sh[w];
   __syncthreads();
   return r;
@@ -697,7 +774,7 @@ __device__ __forceinline__ unsigned long long block_scan_excl64(unsigned long lo
 
 template <class Count>
 __global__ void __launch_bounds__(256) k_scan_tiles(const int32_t* n_ptr, Count cnt, uint32_t* tile_sums) {
-  __shared__ uint32_t sh[16];
+  __shared__ uint32_t sh[kScanSh];
   const int n = ld_acq(n_ptr);
   const int ntiles = (n + kScanTile - 1) / kScanTile;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -713,7 +790,7 @@ __global__ void __launch_bounds__(256) k_scan_tiles(const int32_t* n_ptr, Count 
 }
 
 __global__ void __launch_bounds__(256) k_scan_top(const int32_t* n_ptr, uint32_t* tile_sums) {
-  __shared__ uint32_t sh[16];
+  __shared__ uint32_t sh[kScanSh];
   const int n = ld_acq(n_ptr);
   const int ntiles = (n + kScanTile - 1) / kScanTile;
   uint32_t carry = 0;
@@ -731,7 +808,7 @@ __global__ void __launch_bounds__(256) k_scan_top(const int32_t* n_ptr, uint32_t
 template <class Count>
 __global__ void __launch_bounds__(256) k_scan_apply(const int32_t* n_ptr, Count cnt, const uint32_t* tile_sums,
                                                      uint32_t* out) {
-  __shared__ uint32_t sh[16];
+  __shared__ uint32_t sh[kScanSh];
   const int n = ld_acq(n_ptr);
   const int ntiles = (n + kScanTile - 1) / kScanTile;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -866,7 +943,7 @@ __device__ __forceinline__ int rec_voxel(unsigned long long r) { return (int)(r 
 // per class and workgroup.  The pass counters of the blocks are reset for
 // the next pass, and entries that got no heap block leave the table.
 __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
-  __shared__ unsigned long long sh64[16];
+  __shared__ unsigned long long sh64[kScanSh];
   __shared__ int lcnt[FC_COUNT], gbase[FC_COUNT];
   const int n = ld_acq(&D.ctr->touched_n);
   const int ntiles = (n + kScanTile - 1) / kScanTile;
@@ -1015,61 +1092,79 @@ __device__ __forceinline__ void bitonic_sort_lds(unsigned long long* a, int n) {
   }
 }
 
-// One block of 65..TILE records, one workgroup: a counting sort by voxel in
-// LDS (histogram, scan, scatter) groups every voxel's updates; each record
-// is then placed by its rank (by point) inside its voxel, and each voxel's
-// thread folds its updates in point order.  rin/rout hold TILE records.
-template <class State, int TILE>
+// One block of 65..TILE records, one workgroup of NT threads: a counting
+// sort by voxel in LDS (histogram, scan, scatter) groups every voxel's
+// updates; each record is then placed by its rank (by point) inside its
+// voxel, and each voxel's thread folds its updates in point order.  rin/rout
+// hold TILE records.
+template <class State, int TILE, int NT>
 __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, unsigned long long* rin,
                                                  unsigned long long* rout, uint32_t* cnt, uint32_t* start,
                                                  uint32_t* sh, unsigned long long& uniq) {
+  static_assert(NT == 256 || NT == 512 || NT == 1024, "workgroup of 256, 512 or 1024 threads");
+  constexpr int kPer = TILE / NT;                                         // records per thread
+  constexpr int kVpt = NT >= kVoxelsPerBlock ? 1 : kVoxelsPerBlock / NT;  // voxels per thread
   const int c = d.y;
   const unsigned long long* sg = D.seg + (uint32_t)d.x;
-  for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) cnt[l] = 0;
+  for (int l = threadIdx.x; l < kVoxelsPerBlock; l += NT) cnt[l] = 0;
   // all of this thread's records are requested before any is used (one
   // memory latency per block, not one per record)
-  constexpr int kPer = TILE / 256;
   unsigned long long rr[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int x = threadIdx.x + 256 * u;
+    const int x = threadIdx.x + NT * u;
     rr[u] = x < c ? sg[x] : 0ull;
   }
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int x = threadIdx.x + 256 * u;
+    const int x = threadIdx.x + NT * u;
     if (x < c) {
       rin[x] = rr[u];
       atomicAdd(&cnt[rec_voxel(rr[u])], 1u);
     }
   }
   __syncthreads();
-  // the states of this thread's two voxels (l and l + 256) are loaded now,
-  // so that the loads overlap the sort below
-  static_assert(kVoxelsPerBlock == 512, "two voxels per thread of 256");
+  // the states of this thread's voxels (l = t + NT u) are loaded now, so
+  // that the loads overlap the sort below
   const MapDev& m = D.maps[d.w];
   const float maxW = m.maxW;
   const Codec cd = make_codec(m);
-  const int64_t v0 = (int64_t)d.z * kVoxelsPerBlock + threadIdx.x, v1 = v0 + 256;
-  const bool h0 = cnt[threadIdx.x] != 0, h1 = cnt[threadIdx.x + 256] != 0;
-  float S0 = 0.f, W0 = 0.f, S1 = 0.f, W1 = 0.f;
-  if (h0) State::load(D.heap, v0, cd, S0, W0);
-  if (h1) State::load(D.heap, v1, cd, S1, W1);
+  float S[kVpt], W[kVpt];
+  bool h[kVpt];
+#pragma unroll
+  for (int u = 0; u < kVpt; ++u) {
+    const int l = threadIdx.x + NT * u;
+    h[u] = l < kVoxelsPerBlock && cnt[l] != 0;
+    S[u] = 0.f;
+    W[u] = 0.f;
+    if (h[u]) State::load(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, S[u], W[u]);
+  }
   __syncthreads();
   {
-    const int l0 = 2 * threadIdx.x;
-    const uint32_t c0 = cnt[l0], c1 = cnt[l0 + 1];
+    // the scan takes voxels kVpt t .. kVpt t + kVpt - 1 per thread
+    const int lb = kVpt * threadIdx.x;
+    const bool own = lb < kVoxelsPerBlock;
+    uint32_t cv[kVpt], sum = 0;
+#pragma unroll
+    for (int u = 0; u < kVpt; ++u) {
+      cv[u] = own ? cnt[lb + u] : 0u;
+      sum += cv[u];
+    }
     uint32_t tot;
-    const uint32_t ex = block_scan_excl(c0 + c1, sh, tot);
-    start[l0] = ex;
-    start[l0 + 1] = ex + c0;
-    cnt[l0] = 0;
-    cnt[l0 + 1] = 0;
+    uint32_t run = block_scan_excl(sum, sh, tot);
+    if (own) {
+#pragma unroll
+      for (int u = 0; u < kVpt; ++u) {
+        start[lb + u] = run;
+        run += cv[u];
+        cnt[lb + u] = 0;
+      }
+    }
     if (threadIdx.x == 0) start[kVoxelsPerBlock] = tot;
   }
   __syncthreads();
-  for (int x = threadIdx.x; x < c; x += blockDim.x) {
+  for (int x = threadIdx.x; x < c; x += NT) {
     const unsigned long long r = rin[x];
     const int l = rec_voxel(r);
     rout[start[l] + atomicAdd(&cnt[l], 1u)] = r;
@@ -1080,7 +1175,7 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
   // walk visits a cell once)
   // (eight independent LDS reads per step keep the loop from waiting on
   // each read in turn)
-  for (int x = threadIdx.x; x < c; x += blockDim.x) {
+  for (int x = threadIdx.x; x < c; x += NT) {
     const unsigned long long r = rout[x];
     const int l = rec_voxel(r);
     const int b0 = (int)start[l], b1 = (int)start[l + 1];
@@ -1097,26 +1192,20 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
     rin[b0 + rank] = r;
   }
   __syncthreads();
-  const int l0 = threadIdx.x, l1 = threadIdx.x + 256;
-  const int a0 = (int)start[l0], e0 = (int)start[l0 + 1];
-  const int a1 = (int)start[l1], e1 = (int)start[l1 + 1];
-  for (int i = a0; i < e0; ++i) {
-    float w, wd;
-    weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
-    fold_step(S0, W0, w, wd, maxW);
-  }
-  for (int i = a1; i < e1; ++i) {
-    float w, wd;
-    weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
-    fold_step(S1, W1, w, wd, maxW);
-  }
-  if (h0) {
-    State::store(D.heap, v0, cd, S0, W0);
-    uniq++;
-  }
-  if (h1) {
-    State::store(D.heap, v1, cd, S1, W1);
-    uniq++;
+#pragma unroll
+  for (int u = 0; u < kVpt; ++u) {
+    const int l = threadIdx.x + NT * u;
+    if (l >= kVoxelsPerBlock) continue;
+    const int e = (int)start[l + 1];
+    for (int i = (int)start[l]; i < e; ++i) {
+      float w, wd;
+      weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
+      fold_step(S[u], W[u], w, wd, maxW);
+    }
+    if (h[u]) {
+      State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, S[u], W[u]);
+      uniq++;
+    }
   }
   __syncthreads();
 }
@@ -1149,27 +1238,34 @@ __device__ __forceinline__ void fold_huge_block(const Dev& D, const int4 d, unsi
     __threadfence();
     __syncthreads();
   }
+  // Each tile's run of voxel l is kept as (begin | end << 16), offsets
+  // inside the tile, in registers indexed by compile-time constants only
+  // (an array indexed at run time would live in scratch memory).
+  static_assert(kFoldCap < 65536, "tile offsets in 16 bits");
   for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) {
-    int cur[kMaxChunks], end[kMaxChunks];
+    uint32_t run[kMaxChunks];
     bool any = false;
-    for (int ch = 0; ch < nch; ++ch) {
-      const int lo0 = ch * kFoldCap, hi0 = min(c, lo0 + kFoldCap);
-      int lo = lo0, hi = hi0;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (rec_voxel(ld_acq(&sg[mid])) < l) lo = mid + 1;
-        else hi = mid;
+#pragma unroll
+    for (int ch = 0; ch < kMaxChunks; ++ch) {
+      run[ch] = 0;
+      if (ch < nch) {
+        const int lo0 = ch * kFoldCap, hi0 = min(c, lo0 + kFoldCap);
+        int lo = lo0, hi = hi0;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (rec_voxel(ld_acq(&sg[mid])) < l) lo = mid + 1;
+          else hi = mid;
+        }
+        int lo2 = lo;
+        hi = hi0;
+        while (lo2 < hi) {
+          const int mid = (lo2 + hi) >> 1;
+          if (rec_voxel(ld_acq(&sg[mid])) < l + 1) lo2 = mid + 1;
+          else hi = mid;
+        }
+        run[ch] = (uint32_t)(lo - lo0) | ((uint32_t)(lo2 - lo0) << 16);
+        any = any || (lo2 > lo);
       }
-      int lo2 = lo;
-      hi = hi0;
-      while (lo2 < hi) {
-        const int mid = (lo2 + hi) >> 1;
-        if (rec_voxel(ld_acq(&sg[mid])) < l + 1) lo2 = mid + 1;
-        else hi = mid;
-      }
-      cur[ch] = lo;
-      end[ch] = lo2;
-      any = any || (lo2 > lo);
     }
     if (!any) continue;
     const int64_t vi = (int64_t)slot * kVoxelsPerBlock + l;
@@ -1178,9 +1274,11 @@ __device__ __forceinline__ void fold_huge_block(const Dev& D, const int4 d, unsi
     for (;;) {
       int best = -1;
       unsigned long long bv = ~0ull;
-      for (int ch = 0; ch < nch; ++ch) {
-        if (cur[ch] < end[ch]) {
-          const unsigned long long v = ld_acq(&sg[cur[ch]]);
+#pragma unroll
+      for (int ch = 0; ch < kMaxChunks; ++ch) {
+        const uint32_t b = run[ch] & 0xFFFFu, e = run[ch] >> 16;
+        if (b < e) {
+          const unsigned long long v = ld_acq(&sg[ch * kFoldCap + (int)b]);
           if (best < 0 || v < bv) {
             best = ch;
             bv = v;
@@ -1191,7 +1289,8 @@ __device__ __forceinline__ void fold_huge_block(const Dev& D, const int4 d, unsi
       float w, wd;
       weight_of(__uint_as_float((uint32_t)bv), w, wd);
       fold_step(S, W, w, wd, maxW);
-      cur[best]++;
+#pragma unroll
+      for (int ch = 0; ch < kMaxChunks; ++ch) run[ch] += ch == best ? 1u : 0u;
     }
     State::store(D.heap, vi, cd, S, W);
     uniq++;
@@ -1200,18 +1299,19 @@ __device__ __forceinline__ void fold_huge_block(const Dev& D, const int4 d, unsi
 }
 
 // Fold, first launch.  Work items, in this order: the blocks of 257..1024
-// records, those of 65..256 (a workgroup each: 20 KB of LDS, so up to eight
-// workgroups share a CU), then chunks of the small classes (one group of
-// 64/G blocks per wave).  Workgroup q takes items q, q + grid, ...; with a
+// records, those of 65..256 (a workgroup each), then chunks of the small
+// classes (one group of 64/G blocks per wave).  20.5 KB of LDS and at most
+// 72 VGPRs per thread let seven workgroups share a CU (the second launch
+// bound: seven waves per SIMD).  Workgroup q takes items q, q + grid, ...; with a
 // grid larger than what fits on the chip at once, the dispatcher starts a
 // new workgroup whenever one finishes, which balances the uneven items.
 template <class State>
-__global__ void __launch_bounds__(256) k_fold_a(Dev D) {
+__global__ void __launch_bounds__(256, 7) k_fold_a(Dev D) {
   __shared__ unsigned long long rin[kCountTileS];
   __shared__ unsigned long long rout[kCountTileS];
   __shared__ uint32_t cnt[kVoxelsPerBlock];
   __shared__ uint32_t start[kVoxelsPerBlock + 1];
-  __shared__ uint32_t sh[16];
+  __shared__ uint32_t sh[kScanSh];
   unsigned long long uniq = 0;
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG]);
   const int n_wgl = ld_acq(&D.ctr->fold_n[FC_WGL]);
@@ -1227,9 +1327,9 @@ __global__ void __launch_bounds__(256) k_fold_a(Dev D) {
   const int4* dwgl = D.fold_desc + D.fold_base[FC_WGL];
   for (int it = blockIdx.x; it < total; it += gridDim.x) {
     if (it < n_wg) {
-      fold_count_block<State, kCountTileS>(D, dwg[it], rin, rout, cnt, start, sh, uniq);
+      fold_count_block<State, kCountTileS, 256>(D, dwg[it], rin, rout, cnt, start, sh, uniq);
     } else if (it < n_wg + n_wgl) {
-      fold_count_block<State, kCountTileS>(D, dwgl[it - n_wg], rin, rout, cnt, start, sh, uniq);
+      fold_count_block<State, kCountTileS, 256>(D, dwgl[it - n_wg], rin, rout, cnt, start, sh, uniq);
     } else {
       int c = it - n_wg - n_wgl;
       if (c < c64) {
@@ -1251,18 +1351,21 @@ __global__ void __launch_bounds__(256) k_fold_a(Dev D) {
 // Fold, second launch: blocks of kCountTileS+1..kCountTile records, then the
 // larger ones; one 64 KB LDS buffer serves as the counting tile pair or the
 // sort tile.
+// 1024 threads; eight waves per SIMD (two workgroups, and their 2 x 68 KB
+// of LDS, per CU) caps the kernel at 64 VGPRs.
+constexpr int kFoldBThreads = 1024;
 template <class State>
-__global__ void __launch_bounds__(256) k_fold_b(Dev D) {
+__global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
   static_assert(2 * kCountTile <= kFoldCap, "counting tiles share the sort buffer");
   __shared__ unsigned long long buf[kFoldCap];
   __shared__ uint32_t cnt[kVoxelsPerBlock];
   __shared__ uint32_t start[kVoxelsPerBlock + 1];
-  __shared__ uint32_t sh[16];
+  __shared__ uint32_t sh[kScanSh];
   unsigned long long uniq = 0, dropped = 0;
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG2]);
   const int4* desc = D.fold_desc + D.fold_base[FC_WG2];
   for (int q = blockIdx.x; q < n_wg; q += gridDim.x)
-    fold_count_block<State, kCountTile>(D, desc[q], buf, buf + kCountTile, cnt, start, sh, uniq);
+    fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[q], buf, buf + kCountTile, cnt, start, sh, uniq);
   const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
   const int4* hdesc = D.fold_desc + D.fold_base[FC_HUGE];
   for (int q = blockIdx.x; q < n_huge; q += gridDim.x) fold_huge_block<State>(D, hdesc[q], buf, uniq, dropped);

@@ -415,7 +415,7 @@ int enqueue_pass(tsdf_engine* E, int64_t P, int tid) {
   with_state(E->state_format, [&](auto s) {
     using State = decltype(s);
     hipLaunchKernelGGL(k_fold_a<State>, dim3(fg), dim3(256), 0, st, D);
-    hipLaunchKernelGGL(k_fold_b<State>, dim3(512), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_fold_b<State>, dim3(512), dim3(kFoldBThreads), 0, st, D);
     return 0;
   });
   CHK(timing_mark(E, tid, 5));
