@@ -130,6 +130,18 @@ int tsdf_map_num_blocks(tsdf_map* map, int64_t* out);
  * ({sdf, weight}, local index x + 8y + 64z as in H:507-512).  Writes at most
  * capacity blocks; *count receives the total.  Either output may be NULL. */
 int tsdf_map_export(tsdf_map* map, int32_t* keys, float* voxels, int64_t capacity, int64_t* count);
+/* No reference counterpart (the reference has no snapshot path): restores
+ * blocks from host arrays in the tsdf_map_export layout, so that a map
+ * exported and destroyed to free HBM can be rebuilt later.  Keys must be
+ * distinct, each coordinate within +-8192 (the block range of the point
+ * validity rule).  Each block is found or inserted in the map's hash, takes
+ * a heap block when new (subject to the free heap and the map's block_quota)
+ * and has all 512 voxels overwritten; in a packed state format the values
+ * are re-encoded.  *imported receives the blocks stored, *failed those
+ * refused for lack of heap, quota or hash space (either may be NULL).
+ * Synchronous. */
+int tsdf_map_import(tsdf_map* map, const int32_t* keys, const float* voxels, int64_t n, int64_t* imported,
+                    int64_t* failed);
 
 /* ---- per-scan path ---------------------------------------------------------
  * Replaces: TSDFHandler::allocateVoxelBlocksAndUpdateVoxels (H:1027-1060),
@@ -266,6 +278,34 @@ int tsdf_copy_to_host(tsdf_engine* eng, void* dst, const void* src, int64_t byte
 /* Device-to-device copy with a plain streaming kernel (bytes: multiple of 16);
  * synchronous.  Also serves as the known-size reference for HBM byte counters. */
 int tsdf_device_copy(tsdf_engine* eng, void* dst, const void* src, int64_t bytes);
+
+/* ---- input path ------------------------------------------------------------------
+ * Replaces: Transformer::convertPointCloud2ToPointCloudXYZ
+ * (tsdf_package/src/transformer.cpp:63-67, pcl_conversions::toPCL +
+ * pcl::fromPCLPointCloud2) for the fields the integration reads: the x, y, z
+ * fields of a sensor_msgs/PointCloud2 data block become the pcl::PointXYZ
+ * float4 rows (x, y, z, 1) that tsdf_integrate takes, on the device.
+ * Points are taken row by row (width * height of them); non-finite and
+ * zero-range points are kept and later skipped by the integration, which
+ * counts them in tsdf_scan_stats.skipped. */
+#define TSDF_FIELD_FLOAT32 7 /* sensor_msgs/PointField FLOAT32, the only type PointXYZ fields match */
+typedef struct {
+  int64_t width, height;     /* PointCloud2 width and height */
+  int64_t row_step;          /* bytes per row */
+  int64_t data_bytes;        /* length of the data block (row_step * height in a well-formed message) */
+  int32_t point_step;        /* bytes per point */
+  int32_t x_offset, y_offset, z_offset; /* PointField offsets of x, y, z */
+  int32_t datatype;          /* PointField datatype of x, y, z: TSDF_FIELD_FLOAT32 */
+  int32_t is_bigendian;      /* PointCloud2 is_bigendian */
+} tsdf_cloud2_layout;
+/* cloud: the data block, in device memory when cloud_on_device is nonzero
+ * (then asynchronous on the engine stream), else in host memory (copied
+ * first; synchronous).  points_dev receives width * height float4 rows
+ * (capacity rows at most: TSDF_ERR_CAPACITY beyond); *n_points their number.
+ * A layout whose fields or rows reach past data_bytes, or fields of another
+ * datatype, is refused with TSDF_ERR_INVALID_ARG. */
+int tsdf_cloud2_to_points(tsdf_engine* eng, const tsdf_cloud2_layout* layout, const void* cloud, int cloud_on_device,
+                          float* points_dev, int64_t capacity, int64_t* n_points);
 
 /* Query of the derived constants initGlobalVars computes (H:1296-1315):
  * out_f = {half_voxel, block_size, half_block, block_eps, voxel_eps},

@@ -22,6 +22,8 @@ TSDF_STATE_F32 = 0
 TSDF_STATE_F16 = 1
 TSDF_STATE_F16_QW = 2
 
+TSDF_FIELD_FLOAT32 = 7
+
 STAGES = ["traverse", "assign", "scan", "scatter", "fold", "publish", "gc", "total"]
 
 
@@ -81,6 +83,12 @@ class ServeStats(C.Structure):
                 ("wall_s", C.c_double), ("busy_s", C.c_double), ("points", C.c_int64)]
 
 
+class Cloud2Layout(C.Structure):
+    _fields_ = [("width", C.c_int64), ("height", C.c_int64), ("row_step", C.c_int64), ("data_bytes", C.c_int64),
+                ("point_step", C.c_int32), ("x_offset", C.c_int32), ("y_offset", C.c_int32), ("z_offset", C.c_int32),
+                ("datatype", C.c_int32), ("is_bigendian", C.c_int32)]
+
+
 vp = C.c_void_p
 fp = C.POINTER(C.c_float)
 i32p = C.POINTER(C.c_int32)
@@ -97,6 +105,7 @@ SIGNATURES = {
     "tsdf_map_destroy": [vp],
     "tsdf_map_num_blocks": [vp, i64p],
     "tsdf_map_export": [vp, i32p, fp, C.c_int64, i64p],
+    "tsdf_map_import": [vp, i32p, fp, C.c_int64, i64p, i64p],
     "tsdf_integrate": [vp, vp, C.c_int64, fp],
     "tsdf_publish": [vp, fp, fp, fp, C.c_int64, i64p],
     "tsdf_publish_msg": [vp, fp, fp, C.c_int64, i64p],
@@ -119,6 +128,7 @@ SIGNATURES = {
     "tsdf_copy_to_host": [vp, vp, vp, C.c_int64],
     "tsdf_device_copy": [vp, vp, vp, C.c_int64],
     "tsdf_derived_params": [C.POINTER(Params), fp, i32p],
+    "tsdf_cloud2_to_points": [vp, C.POINTER(Cloud2Layout), vp, C.c_int, vp, C.c_int64, i64p],
 }
 SPECIAL_RESTYPE = {"tsdf_last_error": C.c_char_p, "tsdf_engine_stream": vp}
 SPECIAL_ARGS = {"tsdf_last_error": [], "tsdf_engine_stream": [vp]}

@@ -1500,6 +1500,59 @@ __global__ void __launch_bounds__(256) k_export(Dev D, int map_slot, int32_t* ke
   }
 }
 
+// ---- import (snapshot restore: the inverse of k_export) -------------------------------------
+// Block q of the snapshot is found or inserted in the map's hash; a new entry
+// takes a heap block under the same pool and quota rules as integration
+// (an entry that gets none leaves the table again).  Then all 512 voxels of
+// the block are overwritten.  cnt[0] counts stored blocks, cnt[1] refused ones.
+// Keys are distinct (checked by the host), so no two workgroups share an entry.
+template <class State>
+__global__ void __launch_bounds__(256) k_import(Dev D, int map_slot, const int32_t* keys, const float2* vox, int64_t n,
+                                                int* cnt) {
+  __shared__ int slot_sh;
+  const MapDev m = D.maps[map_slot];
+  const Codec cd = make_codec(m);
+  for (int64_t q = blockIdx.x; q < n; q += gridDim.x) {
+    if (threadIdx.x == 0) {
+      int slot = -1;
+      bool ins = false;
+      const int e = find_or_insert(D, m, pack_key(keys[3 * q], keys[3 * q + 1], keys[3 * q + 2]), ins);
+      if (e >= 0 && !ins) {
+        slot = D.vals[e];
+      } else if (e >= 0) {
+        const int held = atomicAdd(&D.mctr[map_slot].blocks, 1);
+        bool ok = !(m.block_quota > 0 && held >= m.block_quota);
+        if (ok) {
+          const int idx = atomicSub(&D.ctr->heap_top, 1) - 1;
+          if (idx < 0) {
+            atomicAdd(&D.ctr->heap_top, 1);
+            ok = false;
+          } else {
+            slot = D.heap_free[idx];
+          }
+        }
+        if (ok) {
+          D.vals[e] = slot;
+        } else {
+          atomicSub(&D.mctr[map_slot].blocks, 1);
+          D.vals[e] = -1;
+          atomicExch(&D.keys[e], kEmptyKey);
+        }
+      }
+      atomicAdd(&cnt[slot >= 0 ? 0 : 1], 1);
+      slot_sh = slot;
+    }
+    __syncthreads();
+    const int slot = slot_sh;
+    if (slot >= 0)
+      for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) {
+        const float2 v = vox[q * kVoxelsPerBlock + l];
+        State::store(D.heap, (int64_t)slot * kVoxelsPerBlock + l, cd, v.x, v.y);
+      }
+    __syncthreads();
+  }
+}
+
 // ---- garbage collection -----------------------------------------------------------------
 // mode 0: remove blocks beyond the gc radius (H:836-868); mode 1: remove all
 // (map teardown).
@@ -1593,6 +1646,30 @@ __global__ void __launch_bounds__(256) k_gc_chains(Dev D, int map_slot) {
   }
 }
 
+
+// ---- input path: sensor_msgs/PointCloud2 -> pcl::PointXYZ layout -------------------
+// pcl::fromPCLPointCloud2 (called by Transformer::convertPointCloud2ToPointCloudXYZ,
+// tsdf_package/src/transformer.cpp:63-67) copies the FLOAT32 x, y, z fields
+// of every point into PointXYZ {x, y, z, 1}.  Point i of a width x height
+// cloud sits at row i / width, column i % width: byte row * row_step +
+// column * point_step.  Non-finite values are copied as they are.
+__device__ __forceinline__ float cloud_field(const uint8_t* p, bool swap, bool aligned) {
+  uint32_t u;
+  if (aligned) u = *reinterpret_cast<const uint32_t*>(p);
+  else u = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+  if (swap) u = __builtin_bswap32(u);
+  return __uint_as_float(u);
+}
+template <bool Aligned>
+__global__ void __launch_bounds__(256) k_cloud2(const uint8_t* __restrict__ cloud, int64_t width, int64_t n, int64_t row_step,
+                                                int point_step, int xo, int yo, int zo, int swap, float4* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / width, c = i - r * width;
+    const uint8_t* q = cloud + r * row_step + c * point_step;
+    out[i] = make_float4(cloud_field(q + xo, swap, Aligned), cloud_field(q + yo, swap, Aligned),
+                         cloud_field(q + zo, swap, Aligned), 1.0f);
+  }
+}
 
 // ---- plain device copy (also the reference point for HBM byte counters) ------------
 __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {

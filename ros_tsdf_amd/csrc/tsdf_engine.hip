@@ -896,6 +896,61 @@ int tsdf_map_export(tsdf_map* m, int32_t* keys, float* voxels, int64_t capacity,
   return TSDF_OK;
 }
 
+int tsdf_map_import(tsdf_map* m, const int32_t* keys, const float* voxels, int64_t n, int64_t* imported,
+                    int64_t* failed) {
+  CHK(check_map(m));
+  if (n < 0 || (n > 0 && (!keys || !voxels))) return fail(TSDF_ERR_INVALID_ARG, "bad argument");
+  if (imported) *imported = 0;
+  if (failed) *failed = 0;
+  if (n == 0) return TSDF_OK;
+  // keys: inside the block range of the point validity rule, and distinct
+  constexpr int32_t kMaxBlockCoord = 1 << 13;
+  std::vector<uint64_t> packed((size_t)n);
+  for (int64_t q = 0; q < n; ++q) {
+    for (int k = 0; k < 3; ++k)
+      if (keys[3 * q + k] < -kMaxBlockCoord || keys[3 * q + k] > kMaxBlockCoord)
+        return fail(TSDF_ERR_INVALID_ARG, "block %lld: coordinate %d out of range", (long long)q, keys[3 * q + k]);
+    packed[q] = pack_key(keys[3 * q], keys[3 * q + 1], keys[3 * q + 2]);
+  }
+  std::sort(packed.begin(), packed.end());
+  if (std::adjacent_find(packed.begin(), packed.end()) != packed.end())
+    return fail(TSDF_ERR_INVALID_ARG, "duplicate block keys");
+  tsdf_engine* E = m->eng;
+  CHK(set_device(E));
+  hipStream_t st = E->stream;
+  int32_t* dk = nullptr;
+  float2* dv = nullptr;
+  int* dc = nullptr;
+  CHK(dev_alloc(&dk, n * 3));
+  int r = dev_alloc(&dv, n * kVoxelsPerBlock);
+  if (!r) r = dev_alloc(&dc, 2);
+  hipError_t e1 = hipSuccess;
+  int hc[2] = {0, 0};
+  if (!r) {
+    e1 = hipMemcpyAsync(dk, keys, sizeof(int32_t) * 3 * n, hipMemcpyHostToDevice, st);
+    if (e1 == hipSuccess) e1 = hipMemcpyAsync(dv, voxels, sizeof(float2) * kVoxelsPerBlock * n, hipMemcpyHostToDevice, st);
+    if (e1 == hipSuccess) e1 = hipMemsetAsync(dc, 0, sizeof(int) * 2, st);
+    if (e1 == hipSuccess) {
+      Dev D = E->view();
+      with_state(E->state_format, [&](auto s) {
+        hipLaunchKernelGGL(k_import<decltype(s)>, dim3(grid_for(n, 1, 8192)), dim3(256), 0, st, D, m->slot, dk, dv, n, dc);
+        return 0;
+      });
+      e1 = hipGetLastError();
+    }
+    if (e1 == hipSuccess) e1 = hipMemcpyAsync(hc, dc, sizeof(hc), hipMemcpyDeviceToHost, st);
+    if (e1 == hipSuccess) e1 = hipStreamSynchronize(st);
+  }
+  hipFree(dk);
+  hipFree(dv);
+  hipFree(dc);
+  if (r) return r;
+  if (e1 != hipSuccess) return fail(TSDF_ERR_HIP, "import failed: %s", hipGetErrorString(e1));
+  if (imported) *imported = hc[0];
+  if (failed) *failed = hc[1];
+  return TSDF_OK;
+}
+
 int tsdf_integrate(tsdf_map* m, const float* points_dev, int64_t n, const float sensor[3]) {
   CHK(check_map(m));
   if (!sensor) return fail(TSDF_ERR_INVALID_ARG, "null sensor");
@@ -1197,6 +1252,66 @@ int tsdf_device_copy(tsdf_engine* E, void* dst, const void* src, int64_t bytes) 
                      reinterpret_cast<float4*>(dst), n);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(E->stream));
+  return TSDF_OK;
+}
+
+// ---- input path --------------------------------------------------------------------------
+int tsdf_cloud2_to_points(tsdf_engine* E, const tsdf_cloud2_layout* L, const void* cloud, int cloud_on_device,
+                          float* points_dev, int64_t capacity, int64_t* n_points) {
+  if (!E || !L || !n_points) return fail(TSDF_ERR_INVALID_ARG, "null argument");
+  if (L->width < 0 || L->height < 0 || L->point_step <= 0 || L->row_step < 0 || L->data_bytes < 0)
+    return fail(TSDF_ERR_INVALID_ARG, "negative cloud size or non-positive point_step");
+  if (L->datatype != TSDF_FIELD_FLOAT32)
+    return fail(TSDF_ERR_INVALID_ARG, "x/y/z datatype %d: only FLOAT32 (7) fields convert to pcl::PointXYZ", L->datatype);
+  if (L->is_bigendian != 0 && L->is_bigendian != 1) return fail(TSDF_ERR_INVALID_ARG, "is_bigendian must be 0 or 1");
+  const int32_t off[3] = {L->x_offset, L->y_offset, L->z_offset};
+  for (int k = 0; k < 3; ++k)
+    if (off[k] < 0 || (int64_t)off[k] + 4 > L->point_step)
+      return fail(TSDF_ERR_INVALID_ARG, "field offset %d does not fit in point_step %d", off[k], L->point_step);
+  if (L->width > 0 && L->height > 1 && L->row_step < L->width * (int64_t)L->point_step)
+    return fail(TSDF_ERR_INVALID_ARG, "row_step %lld below width * point_step", (long long)L->row_step);
+  if (L->width > 0 && L->height > 0 && L->width > (int64_t{1} << 31) / L->height)
+    return fail(TSDF_ERR_INVALID_ARG, "cloud of more than 2^31 points");
+  if (L->height > 1 && L->height - 1 > L->data_bytes / std::max<int64_t>(L->row_step, 1))
+    return fail(TSDF_ERR_INVALID_ARG, "rows reach past the %lld-byte data block", (long long)L->data_bytes);
+  const int64_t n = L->width * L->height;
+  *n_points = n;
+  if (n == 0) return TSDF_OK;
+  if (n > capacity) return fail(TSDF_ERR_CAPACITY, "cloud of %lld points, capacity %lld", (long long)n, (long long)capacity);
+  if (!cloud || !points_dev) return fail(TSDF_ERR_INVALID_ARG, "null cloud or output");
+  // bytes read: up to the end of the last point of the last row
+  const int64_t extent = (L->height - 1) * L->row_step + L->width * (int64_t)L->point_step;
+  if (extent > L->data_bytes)
+    return fail(TSDF_ERR_INVALID_ARG, "layout reads %lld bytes of a %lld-byte data block", (long long)extent,
+                (long long)L->data_bytes);
+  CHK(set_device(E));
+  const uint8_t* src = static_cast<const uint8_t*>(cloud);
+  uint8_t* staged = nullptr;
+  if (!cloud_on_device) {
+    CHK(dev_alloc(&staged, extent));
+    const hipError_t ce = hipMemcpyAsync(staged, cloud, (size_t)extent, hipMemcpyHostToDevice, E->stream);
+    if (ce != hipSuccess) {
+      hipFree(staged);
+      return fail(TSDF_ERR_HIP, "H2D copy failed: %s", hipGetErrorString(ce));
+    }
+    src = staged;
+  }
+  const bool aligned = (reinterpret_cast<uintptr_t>(src) % 4) == 0 && (L->point_step % 4) == 0 &&
+                       (L->row_step % 4) == 0 && (off[0] % 4) == 0 && (off[1] % 4) == 0 && (off[2] % 4) == 0;
+  const dim3 grid(grid_for(n, 256, 8192));
+  if (aligned)
+    hipLaunchKernelGGL(k_cloud2<true>, grid, dim3(256), 0, E->stream, src, L->width, n, L->row_step, L->point_step,
+                       off[0], off[1], off[2], L->is_bigendian, reinterpret_cast<float4*>(points_dev));
+  else
+    hipLaunchKernelGGL(k_cloud2<false>, grid, dim3(256), 0, E->stream, src, L->width, n, L->row_step, L->point_step,
+                       off[0], off[1], off[2], L->is_bigendian, reinterpret_cast<float4*>(points_dev));
+  hipError_t ke = hipGetLastError();
+  if (staged) {
+    const hipError_t se = hipStreamSynchronize(E->stream);
+    if (ke == hipSuccess) ke = se;
+    hipFree(staged);
+  }
+  if (ke != hipSuccess) return fail(TSDF_ERR_HIP, "cloud decode failed: %s", hipGetErrorString(ke));
   return TSDF_OK;
 }
 

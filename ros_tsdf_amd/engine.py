@@ -123,6 +123,26 @@ class Engine:
     def buffer(self, nbytes) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
 
+    def cloud2_to_points(self, data, width, height, point_step, row_step, offsets, out: DeviceBuffer,
+                         is_bigendian=False, datatype=abi.TSDF_FIELD_FLOAT32, data_bytes=None) -> int:
+        """A sensor_msgs/PointCloud2 data block (host uint8 array or DeviceBuffer) -> float4 points in out.
+
+        offsets: the byte offsets of the x, y, z fields.  Returns the number of points written
+        (width * height).  Replaces Transformer::convertPointCloud2ToPointCloudXYZ
+        (tsdf_package/src/transformer.cpp:63-67)."""
+        on_dev = isinstance(data, DeviceBuffer)
+        if on_dev:
+            ptr, nbytes = data.ptr, data.nbytes
+        else:
+            data = np.ascontiguousarray(data, dtype=np.uint8)
+            ptr, nbytes = data.ctypes.data, data.nbytes
+        lay = abi.Cloud2Layout(width, height, row_step, nbytes if data_bytes is None else data_bytes, point_step,
+                               offsets[0], offsets[1], offsets[2], datatype, 1 if is_bigendian else 0)
+        n = C.c_int64()
+        check(abi.load().tsdf_cloud2_to_points(self.h, C.byref(lay), C.c_void_p(ptr), 1 if on_dev else 0,
+                                               C.c_void_p(out.ptr), out.nbytes // 16, C.byref(n)))
+        return int(n.value)
+
 
 class Map:
     def __init__(self, engine: Engine, params, buckets_log2=16, overflow_buckets=0, max_chain=64, block_quota=0,
@@ -190,6 +210,16 @@ class Map:
         order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
         keys, vox = keys[order], vox[order]
         return keys, vox[:, :, 0].copy(), vox[:, :, 1].copy()
+
+    def import_blocks(self, keys, sdf, weight):
+        """Restore blocks in the export layout (keys [n,3], sdf/weight [n,512]); returns (stored, refused)."""
+        keys = np.ascontiguousarray(keys, np.int32).reshape(-1, 3)
+        vox = np.ascontiguousarray(np.stack([np.asarray(sdf, np.float32), np.asarray(weight, np.float32)], axis=-1))
+        assert vox.shape == (keys.shape[0], 512, 2)
+        a, b = C.c_int64(), C.c_int64()
+        check(abi.load().tsdf_map_import(self.h, keys.ctypes.data_as(abi.i32p), _fptr(vox), keys.shape[0],
+                                         C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
 
     def process_scan(self, points: np.ndarray, sensor, capacity=None):
         """The reference's processPointCloudAndUpdateVoxels sequence with host arrays."""
