@@ -83,7 +83,7 @@ struct Dev {
   unsigned long long* keys;  // [entries]
   int32_t* vals;             // [entries] heap block of an entry, -1 none
   unsigned long long* blk_cnt;  // [entries] pass counter: voxel records in the high 32 bits
-  uint32_t* touch_idx;       // [entries] index in the touched list
+  uint32_t* blk_base;        // [entries] first record of the block's segment in this pass
   int32_t* next;             // [buckets] overflow link, -1 none
   int32_t* ovf_stack;        // [buckets] per-map overflow free stacks
   void* heap;                // [heap_blocks][512] voxel state
@@ -98,7 +98,6 @@ struct Dev {
   int64_t pcap;
   uint32_t* touched;         // [entries]
   uint8_t* touched_map;      // [entries] map slot of each touched block
-  uint32_t* seg_off;         // [entries + 1] record offset of each touched block
   uint32_t* tile_sums;       // [entries / 1024 + 2]
   uint32_t* fail_list;       // [entries] inserted entries without a heap block (removed after binning)
   int4* fold_desc;           // per size class c, from fold_base[c]: {record base, records, heap block, map}

@@ -220,7 +220,19 @@ __device__ inline int find_or_insert(const Dev& D, const MapDev& m, unsigned lon
 
 // ---- pass kernels -----------------------------------------------------------------
 // Zeroes the pass counters and the partial slots of the statistics in mask.
-__global__ void __launch_bounds__(256) k_pass_begin(Dev D, uint32_t mask) {
+// A pass of at most kInlineScans scans brings its header and scan
+// descriptors as kernel arguments, and k_pass_begin stores them for the
+// pass's other kernels: no host-to-device copy precedes the pass.  nscans < 0:
+// the descriptors were copied into place already (larger batches, and graph
+// replays, whose captured arguments would be stale).
+constexpr int kInlineScans = 32;
+struct PassInline {
+  PassHdr hdr;
+  int32_t nscans;
+  ScanDev scans[kInlineScans];
+};
+
+__global__ void __launch_bounds__(256) k_pass_begin(Dev D, uint32_t mask, PassInline inl) {
   if (blockIdx.x == 0) {
     const int t = threadIdx.x;
     if (t == 0) {
@@ -228,6 +240,10 @@ __global__ void __launch_bounds__(256) k_pass_begin(Dev D, uint32_t mask) {
       D.ctr->fail_n = 0;
     }
     if (t < FC_COUNT) D.ctr->fold_n[t] = 0;
+    if (inl.nscans >= 0) {
+      if (t == 0) *const_cast<PassHdr*>(D.hdr) = inl.hdr;
+      if (t < inl.nscans) const_cast<ScanDev*>(D.scans)[t] = inl.scans[t];
+    }
   }
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ST_COUNT * kPartSlots; i += gridDim.x * blockDim.x)
     if (mask & (1u << (i / kPartSlots))) D.part[i] = 0;
@@ -584,7 +600,6 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
               const int t = atomicAdd(&D.ctr->touched_n, 1);
               D.touched[t] = (uint32_t)e;
               D.touched_map[t] = (uint8_t)m.slot;
-              D.touch_idx[e] = (uint32_t)t;
             }
             D.pair_e[(int64_t)nj * stride + i] = kPairDirect | (uint32_t)e;
             D.pair_r[(int64_t)nj * stride + i] = (uint32_t)(old >> 32);
@@ -672,7 +687,6 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
       const uint32_t e = ltouch[x];
       D.touched[g_touch + x] = e;
       D.touched_map[g_touch + x] = ltouch_map[x];
-      D.touch_idx[e] = (uint32_t)(g_touch + x);
     }
     // ---- phase C: final (entry, record offset) of every run ----
     if (i < P) {
@@ -844,7 +858,7 @@ __device__ __forceinline__ unsigned long long make_rec(int local, int point, flo
 constexpr int kEmitRuns = 6;
 __device__ __forceinline__ int64_t run_offset(const Dev& D, int r, int i) {
   const uint32_t e = D.pair_e[(int64_t)r * D.pcap + i];
-  return e == kPairFail ? -1 : (int64_t)D.seg_off[D.touch_idx[e]] + D.pair_r[(int64_t)r * D.pcap + i];
+  return e == kPairFail ? -1 : (int64_t)D.blk_base[e] + D.pair_r[(int64_t)r * D.pcap + i];
 }
 
 // Second walk of every point's voxel DDA: each voxel's update record
@@ -855,8 +869,30 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
   const PassHdr hdr = *D.hdr;
   const int P = hdr.total;
   const int64_t stride = D.pcap;
+  // run rows that exist in the pair arrays (they hold hdr.stride rows)
+  const int rows = min(kEmitRuns, hdr.stride);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
-    if (D.pair_n[i] == 0) continue;
+    // absolute record offsets of the point's runs, loaded before the walk
+    // (the first kEmitRuns in registers; later runs, which need a walk far
+    // longer than the truncation band of the workloads, are looked up when
+    // reached).  The run rows are requested together with the run count, so
+    // the blocks' record bases are one dependent load away.
+    const int nj = D.pair_n[i];
+    uint32_t pe[kEmitRuns], pr[kEmitRuns];
+#pragma unroll
+    for (int r = 0; r < kEmitRuns; ++r) {
+      pe[r] = kPairFail;
+      pr[r] = 0;
+      if (r < rows) {
+        pe[r] = D.pair_e[(int64_t)r * stride + i];
+        pr[r] = D.pair_r[(int64_t)r * stride + i];
+      }
+    }
+    if (nj == 0) continue;
+    int64_t roff[kEmitRuns];
+#pragma unroll
+    for (int r = 0; r < kEmitRuns; ++r)
+      roff[r] = r < nj && pe[r] != kPairFail ? (int64_t)D.blk_base[pe[r]] + pr[r] : -1;
     const ScanDev sd = D.scans[find_scan(D.scans, hdr.nscans, i)];
     const MapDev m = D.maps[sd.map];
     const float4 q = hdr.points[i];
@@ -864,14 +900,6 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
     const float s[3] = {sd.s[0], sd.s[1], sd.s[2]};
     float a[3], b[3];
     truncation_segment(p, s, m.T, a, b);
-    // absolute record offsets of the point's runs, loaded before the walk
-    // (the first kEmitRuns in registers; later runs, which need a walk far
-    // longer than the truncation band of the workloads, are looked up when
-    // reached)
-    const int nj = D.pair_n[i];
-    int64_t roff[kEmitRuns];
-#pragma unroll
-    for (int r = 0; r < kEmitRuns; ++r) roff[r] = r < nj ? run_offset(D, r, i) : -1;
     unsigned long long rkey = kEmptyKey;
     int j = -1;
     int64_t at = -1;
@@ -987,7 +1015,7 @@ __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
       rank[k] = 0;
       dsc[k] = make_int4(0, 0, 0, 0);
       if (i0 + k < n) {
-        D.seg_off[i0 + k] = ex;
+        D.blk_base[e4[k]] = ex;
         D.blk_cnt[e4[k]] = 0;
         if (sl4[k] < 0) {
           dropped += cn[k];
@@ -999,7 +1027,6 @@ __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
       }
       ex += cn[k];
     }
-    if (t == ntiles - 1 && threadIdx.x == 0) D.seg_off[n] = carry + (uint32_t)tot;
     __syncthreads();
     if (threadIdx.x < FC_COUNT)
       gbase[threadIdx.x] = lcnt[threadIdx.x] ? atomicAdd(&D.ctr->fold_n[threadIdx.x], lcnt[threadIdx.x]) : 0;

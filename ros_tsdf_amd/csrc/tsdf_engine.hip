@@ -114,7 +114,7 @@ struct tsdf_engine {
   unsigned long long* keys = nullptr;
   int32_t* vals = nullptr;
   unsigned long long* blk_cnt = nullptr;
-  uint32_t* touch_idx = nullptr;
+  uint32_t* blk_base = nullptr;
   int32_t* next = nullptr;
   int32_t* ovf_stack = nullptr;
   void* heap = nullptr;
@@ -135,7 +135,6 @@ struct tsdf_engine {
   unsigned long long* seg = nullptr;
   int64_t seg_cap = 0;
   uint32_t* touched = nullptr;
-  uint32_t* seg_off = nullptr;
   uint32_t* tile_sums = nullptr;
   uint32_t* fail_list = nullptr;
   int4* fold_desc = nullptr;
@@ -183,7 +182,7 @@ struct tsdf_engine {
     D.keys = keys;
     D.vals = vals;
     D.blk_cnt = blk_cnt;
-    D.touch_idx = touch_idx;
+    D.blk_base = blk_base;
     D.next = next;
     D.ovf_stack = ovf_stack;
     D.heap = heap;
@@ -196,7 +195,6 @@ struct tsdf_engine {
     D.pair_n = pair_n;
     D.pcap = pcap;
     D.touched = touched;
-    D.seg_off = seg_off;
     D.tile_sums = tile_sums;
     D.fail_list = fail_list;
     D.fold_desc = fold_desc;
@@ -389,12 +387,13 @@ int stage_descs(tsdf_engine* E, const PassHdr& hdr, const ScanDev* scans, int ns
 }
 
 // Enqueues the pass kernels sized for P points (grid sizes only; the
-// kernels read the actual counts from device memory).
-int enqueue_pass(tsdf_engine* E, int64_t P, int tid) {
+// kernels read the actual counts from device memory).  inl: the pass's
+// descriptors as kernel arguments, or nscans = -1 when they were staged.
+int enqueue_pass(tsdf_engine* E, int64_t P, int tid, const PassInline& inl) {
   Dev D = E->view();
   hipStream_t st = E->stream;
   const tsdf_launch_config& L = E->launch;
-  hipLaunchKernelGGL(k_pass_begin, dim3(64), dim3(256), 0, st, D, kPassStats);
+  hipLaunchKernelGGL(k_pass_begin, dim3(64), dim3(256), 0, st, D, kPassStats, inl);
   CHK(timing_mark(E, tid, 0));
   const int tb = L.traverse_block;
   hipLaunchKernelGGL(k_bin, dim3(grid_for(P, tb, L.traverse_grid_cap)), dim3(tb), 0, st, D);
@@ -470,15 +469,21 @@ int run_batch(tsdf_engine* E, tsdf_map* const* maps, int nmaps, const tsdf_scan_
   if (!contiguous) {
     return fail(TSDF_ERR_INVALID_ARG, "scans of a batch must be contiguous and in order in the point buffer");
   }
-  PassHdr hdr{};
-  hdr.points = pts + base0;
-  hdr.total = (int32_t)P;
-  hdr.nscans = nscans;
-  hdr.stride = stride;
-  CHK(stage_descs(E, hdr, sd, nscans));
+  PassInline inl{};
+  inl.hdr.points = pts + base0;
+  inl.hdr.total = (int32_t)P;
+  inl.hdr.nscans = nscans;
+  inl.hdr.stride = stride;
+  if (nscans <= kInlineScans) {
+    inl.nscans = nscans;
+    std::memcpy(inl.scans, sd, sizeof(ScanDev) * nscans);
+  } else {
+    inl.nscans = -1;
+    CHK(stage_descs(E, inl.hdr, sd, nscans));
+  }
   int tid;
   CHK(timing_begin(E, 0, &tid));
-  CHK(enqueue_pass(E, P, tid));
+  CHK(enqueue_pass(E, P, tid, inl));
   timing_end(E, tid);
   E->last_points = P;
   return TSDF_OK;
@@ -723,11 +728,11 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
   const int64_t entries = E->entries;
   if ((r = dev_alloc(&E->maps_dev, kMaxMaps)) || (r = dev_alloc(&E->mctr, kMaxMaps)) || (r = dev_alloc(&E->ctr, 1)) ||
       (r = dev_alloc(&E->keys, entries)) || (r = dev_alloc(&E->vals, entries)) || (r = dev_alloc(&E->blk_cnt, entries)) ||
-      (r = dev_alloc(&E->touch_idx, entries)) || (r = dev_alloc(&E->next, c.total_buckets)) ||
+      (r = dev_alloc(&E->blk_base, entries)) || (r = dev_alloc(&E->next, c.total_buckets)) ||
       (r = dev_alloc(&E->ovf_stack, c.total_buckets)) ||
       (r = dev_alloc(&E->heap_free, c.heap_blocks)) || (r = dev_alloc(&E->freed, c.heap_blocks)) ||
       (r = dev_alloc(&E->desc_dev, 1)) ||
-      (r = dev_alloc(&E->touched, entries)) || (r = dev_alloc(&E->seg_off, entries + 1)) ||
+      (r = dev_alloc(&E->touched, entries)) ||
       (r = dev_alloc(&E->tile_sums, entries / kScanTile + 2)) || (r = dev_alloc(&E->fail_list, entries)) ||
       (r = dev_alloc(&E->touched_map, entries)) || (r = dev_alloc(&E->part, ST_COUNT * kPartSlots)) ||
       (r = dev_alloc(&E->sel, entries)) || (r = dev_alloc(&E->pub_cnt, entries + 1)) ||
@@ -784,9 +789,9 @@ int tsdf_engine_destroy(tsdf_engine* E) {
       delete E->map_by_slot[i];
       E->map_by_slot[i] = nullptr;
     }
-  void* ptrs[] = {E->maps_dev, E->mctr, E->ctr, E->keys, E->vals, E->blk_cnt, E->touch_idx, E->next, E->ovf_stack,
+  void* ptrs[] = {E->maps_dev, E->mctr, E->ctr, E->keys, E->vals, E->blk_cnt, E->blk_base, E->next, E->ovf_stack,
                   E->heap, E->heap_free, E->freed, E->desc_dev, E->pair_e, E->pair_r, E->pair_n,
-                  E->seg, E->touched, E->seg_off, E->tile_sums, E->fail_list, E->fold_desc, E->part, E->touched_map, E->sel,
+                  E->seg, E->touched, E->tile_sums, E->fail_list, E->fold_desc, E->part, E->touched_map, E->sel,
                   E->pub_cnt, E->pub_off, E->pub_buf};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -1116,7 +1121,9 @@ static int graph_capture(tsdf_batch_graph* g) {
   const bool t = E->timing;
   E->timing = false;
   HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-  int r = enqueue_pass(E, g->max_points, -1);
+  PassInline staged{};
+  staged.nscans = -1;  // a replay's descriptors are copied in before each launch
+  int r = enqueue_pass(E, g->max_points, -1, staged);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(E->stream, &graph);
   E->timing = t;
