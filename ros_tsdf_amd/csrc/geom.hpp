@@ -169,6 +169,13 @@ __device__ __forceinline__ int walk_cells(const float a[3], const float b[3], fl
 __device__ __forceinline__ float cell_centre_of(int32_t k, float scale, float iscale, float half) {
   return div_by_scale((float)k, scale, iscale) + half;
 }
+// The same for a map whose product form is enabled (iscale != 0).
+__device__ __forceinline__ float product_centre(int32_t k, float scale, float iscale, float half) {
+  const float x = (float)k;
+  const float q0 = x * iscale;
+  const float r = __builtin_fmaf(-q0, scale, x);
+  return __builtin_fmaf(r, iscale, q0) + half;
+}
 
 // Block and local index of voxel k.  The reference derives them from the
 // voxel centre in FP32 (H:612, H:622-626, H:507-512): floor(c / (8 L)) and

@@ -854,11 +854,12 @@ __device__ __forceinline__ unsigned long long make_rec(int local, int point, flo
   return ((unsigned long long)local << 55) | ((unsigned long long)point << 32) | (unsigned long long)__float_as_uint(d);
 }
 
-// Record offset of run r of point i, or -1 when its block could not be stored.
+// Record offset of run r of point i, or -1 when its block could not be stored
+// (a pass holds fewer than 2^31 records: kMaxPassPoints times at most 128).
 constexpr int kEmitRuns = 6;
-__device__ __forceinline__ int64_t run_offset(const Dev& D, int r, int i) {
+__device__ __forceinline__ int32_t run_offset(const Dev& D, int r, int i) {
   const uint32_t e = D.pair_e[(int64_t)r * D.pcap + i];
-  return e == kPairFail ? -1 : (int64_t)D.blk_base[e] + D.pair_r[(int64_t)r * D.pcap + i];
+  return e == kPairFail ? -1 : (int32_t)(D.blk_base[e] + D.pair_r[(int64_t)r * D.pcap + i]);
 }
 
 // Second walk of every point's voxel DDA: each voxel's update record
@@ -889,10 +890,10 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
       }
     }
     if (nj == 0) continue;
-    int64_t roff[kEmitRuns];
+    int32_t roff[kEmitRuns];
 #pragma unroll
     for (int r = 0; r < kEmitRuns; ++r)
-      roff[r] = r < nj && pe[r] != kPairFail ? (int64_t)D.blk_base[pe[r]] + pr[r] : -1;
+      roff[r] = r < nj && pe[r] != kPairFail ? (int32_t)(D.blk_base[pe[r]] + pr[r]) : -1;
     const ScanDev sd = D.scans[find_scan(D.scans, hdr.nscans, i)];
     const MapDev m = D.maps[sd.map];
     const float4 q = hdr.points[i];
@@ -900,30 +901,41 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
     const float s[3] = {sd.s[0], sd.s[1], sd.s[2]};
     float a[3], b[3];
     truncation_segment(p, s, m.T, a, b);
-    unsigned long long rkey = kEmptyKey;
+    int32_t pb0 = INT32_MIN, pb1 = 0, pb2 = 0;  // block of the current run
     int j = -1;
-    int64_t at = -1;
+    int32_t at = -1;
     bool tv;
-    walk_cells(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
-      const unsigned long long key = pack_key(block_of(k0), block_of(k1), block_of(k2));
-      if (key != rkey) {
-        rkey = key;
-        ++j;
-        at = j < kEmitRuns ? -1 : run_offset(D, j, i);
+    // kProduct: every lane's map forms k / scale by the checked product (a
+    // product and two FMAs per axis, no branch); otherwise each axis picks
+    // its map's form.  The choice is made once per wave.
+    auto walk = [&](auto product) {
+      constexpr bool kProduct = decltype(product)::value;
+      walk_cells(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
+        const int32_t b0 = block_of(k0), b1 = block_of(k1), b2 = block_of(k2);
+        if (b0 != pb0 || b1 != pb1 || b2 != pb2) {
+          pb0 = b0;
+          pb1 = b1;
+          pb2 = b2;
+          ++j;
+          at = j < kEmitRuns ? -1 : run_offset(D, j, i);
 #pragma unroll
-        for (int r = 0; r < kEmitRuns; ++r)
-          if (r == j) at = roff[r];
-      }
-      if (at >= 0) {
-        const int32_t kk[3] = {k0, k1, k2};
-        // centres from the integer cell (a product and an FMA per axis; no
-        // branches, which cost more than recomputing under divergence)
-        const float vc[3] = {cell_centre_of(k0, m.sc_v, m.isc_v, m.half_vs), cell_centre_of(k1, m.sc_v, m.isc_v, m.half_vs),
-                             cell_centre_of(k2, m.sc_v, m.isc_v, m.half_vs)};
-        D.seg[at++] = make_rec(local_index(kk), i, signed_distance(p, s, vc));
-      }
-      return true;
-    });
+          for (int r = 0; r < kEmitRuns; ++r)
+            if (r == j) at = roff[r];
+        }
+        if (at >= 0) {
+          const int32_t kk[3] = {k0, k1, k2};
+          float vc[3];
+#pragma unroll
+          for (int u = 0; u < 3; ++u)
+            vc[u] = kProduct ? product_centre(kk[u], m.sc_v, m.isc_v, m.half_vs)
+                             : cell_centre_of(kk[u], m.sc_v, m.isc_v, m.half_vs);
+          D.seg[(uint32_t)at++] = make_rec(local_index(kk), i, signed_distance(p, s, vc));
+        }
+        return true;
+      });
+    };
+    if (__ballot(m.isc_v == 0.0f) == 0) walk(std::true_type{});
+    else walk(std::false_type{});
   }
 }
 
