@@ -43,7 +43,7 @@ constexpr int kScanTile = 1024;  // elements per workgroup in the scan kernels
 // Size classes of touched blocks (records per block): the fold kernel and
 // the number of blocks folded together differ per class.
 enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WGL, FC_WG, FC_WG2, FC_HUGE, FC_COUNT };
-constexpr int kWgSplit = 256;  // FC_WGL: 65..256 records, FC_WG: 257..kCountTileS
+constexpr int kWgSplit = 128;  // FC_WGL: 65..128 records (a wave each), FC_WG: 129..kCountTileS
 static_assert(FC_COUNT <= (int)(sizeof(EngCtr::fold_n) / sizeof(int32_t)), "fold class counters");
 
 // Descriptors a class can need when a pass holds at most `blocks` touched
@@ -1113,6 +1113,86 @@ __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int 
   }
 }
 
+// One block of 65..64 R records, one wave (R records per lane):
+// the records are bitonic-sorted in registers across the wave (element
+// r * 64 + lane), which orders them by voxel, then by point; the sorted
+// records go to the wave's LDS strip (64 R entries), and the lane holding
+// the first record of each voxel folds that voxel's run from the strip.
+// No workgroup barrier: the wave only waits for its own LDS traffic.
+template <class State, int R>
+__device__ __forceinline__ void fold_wave_block(const Dev& D, const int4 d, unsigned long long* strip,
+                                                unsigned long long& uniq) {
+  constexpr int N = 64 * R;
+  const int lane = lane_id();
+  const int n = d.y;
+  const unsigned long long* sg = D.seg + (uint32_t)d.x;
+  unsigned long long v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = 64 * r + lane < n ? sg[64 * r + lane] : ~0ull;
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+        // partner in another register of this lane; the direction depends
+        // on the register only (k > 64)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (r & (j >> 6)) continue;
+          const int r2 = r | (j >> 6);
+          const bool up = ((64 * r) & k) == 0;
+          const unsigned long long a = v[r], b = v[r2];
+          const bool sw = up ? a > b : a < b;
+          v[r] = sw ? b : a;
+          v[r2] = sw ? a : b;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const unsigned long long o = __shfl_xor(v[r], j);
+          const bool up = (((64 * r) | lane) & k) == 0;
+          const bool keep_min = ((lane & j) == 0) == up;
+          v[r] = keep_min ? (o < v[r] ? o : v[r]) : (o > v[r] ? o : v[r]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) strip[64 * r + lane] = v[r];
+  wave_sync();
+  const MapDev& m = D.maps[d.w];
+  const float maxW = m.maxW;
+  const Codec cd = make_codec(m);
+  bool head[R];
+  float S[R], W[R];
+  // heads and their voxels' states first (the state loads overlap each other)
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = 64 * r + lane;
+    head[r] = e < n && (e == 0 || rec_voxel(strip[e - 1]) != rec_voxel(v[r]));
+    S[r] = 0.f;
+    W[r] = 0.f;
+    if (head[r]) State::load(D.heap, (int64_t)d.z * kVoxelsPerBlock + rec_voxel(v[r]), cd, S[r], W[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (!head[r]) continue;
+    const int l = rec_voxel(v[r]);
+    int x = 64 * r + lane;
+    unsigned long long cur = v[r];
+    for (;;) {
+      float w, wd;
+      weight_of(__uint_as_float((uint32_t)cur), w, wd);
+      fold_step(S[r], W[r], w, wd, maxW);
+      if (++x >= n) break;
+      cur = strip[x];
+      if (rec_voxel(cur) != l) break;
+    }
+    State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, S[r], W[r]);
+    uniq++;
+  }
+}
+
 __device__ __forceinline__ void bitonic_sort_lds(unsigned long long* a, int n) {
   for (int k = 2; k <= n; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
@@ -1337,9 +1417,9 @@ __device__ __forceinline__ void fold_huge_block(const Dev& D, const int4 d, unsi
   __syncthreads();
 }
 
-// Fold, first launch.  Work items, in this order: the blocks of 257..1024
-// records, those of 65..256 (a workgroup each), then chunks of the small
-// classes (one group of 64/G blocks per wave).  20.5 KB of LDS and at most
+// Fold, first launch.  Work items, in this order: the blocks of 129..1024
+// records (a workgroup each), those of 65..128 (a wave each), then chunks of
+// the small classes (one group of 64/G blocks per wave).  20.5 KB of LDS and at most
 // 72 VGPRs per thread let seven workgroups share a CU (the second launch
 // bound: seven waves per SIMD).  Workgroup q takes items q, q + grid, ...; with a
 // grid larger than what fits on the chip at once, the dispatcher starts a
@@ -1358,19 +1438,28 @@ __global__ void __launch_bounds__(256, 7) k_fold_a(Dev D) {
   const int n32 = ld_acq(&D.ctr->fold_n[FC_32]);
   const int n16 = ld_acq(&D.ctr->fold_n[FC_16]);
   const int waves = blockDim.x >> 6, wave = threadIdx.x >> 6;
-  // blocks per small-class item: one group of 64/G blocks per wave
-  const int k64 = waves * 1, k32 = waves * 2, k16 = waves * 4;
+  // blocks per wave-level item: one block of 65..128 records per wave; one
+  // group of 64/G small blocks per wave
+  const int kl = waves, k64 = waves * 1, k32 = waves * 2, k16 = waves * 4;
+  const int cl = (n_wgl + kl - 1) / kl;
   const int c64 = (n64 + k64 - 1) / k64, c32 = (n32 + k32 - 1) / k32, c16 = (n16 + k16 - 1) / k16;
-  const int total = n_wg + n_wgl + c64 + c32 + c16;
+  const int total = n_wg + cl + c64 + c32 + c16;
   const int4* dwg = D.fold_desc + D.fold_base[FC_WG];
   const int4* dwgl = D.fold_desc + D.fold_base[FC_WGL];
+  static_assert(kCountTileS >= 4 * kWgSplit, "a wave's strip of kWgSplit records in rin");
   for (int it = blockIdx.x; it < total; it += gridDim.x) {
     if (it < n_wg) {
       fold_count_block<State, kCountTileS, 256>(D, dwg[it], rin, rout, cnt, start, sh, uniq);
-    } else if (it < n_wg + n_wgl) {
-      fold_count_block<State, kCountTileS, 256>(D, dwgl[it - n_wg], rin, rout, cnt, start, sh, uniq);
+    } else if (it < n_wg + cl) {
+      const int q = (it - n_wg) * kl + wave;
+      if (q < n_wgl) {
+        const int4 d = dwgl[q];
+        unsigned long long* strip = rin + wave * kWgSplit;
+        fold_wave_block<State, kWgSplit / 64>(D, d, strip, uniq);
+      }
+      __syncthreads();  // the strips live in rin, which the next item may use
     } else {
-      int c = it - n_wg - n_wgl;
+      int c = it - n_wg - cl;
       if (c < c64) {
         const int q = c * k64 + wave;
         fold_small_range<State, 64>(D, q, min(q + 1, n64), uniq);
