@@ -130,7 +130,7 @@ def main():
     kernel_stages = ["traverse", "assign", "scan", "scatter", "fold"]
     dom = max(kernel_stages, key=lambda k: stage_ms[k])
     # stage -> the launches between its two timing events ("*": a group of launches)
-    dom_kernel = {"traverse": "k_bin", "assign": "k_bin", "scan": "k_scan_tiles+k_seg_apply", "scatter": "k_emit",
+    dom_kernel = {"traverse": "k_bin", "assign": "k_bin", "scan": "k_pass_scan+k_seg_apply", "scatter": "k_emit",
                   "fold": "k_fold_a+k_fold_b"}[dom]
     achieved = alg_bytes / (stage_ms[dom] * 1e-3)
     traffic = None
@@ -150,7 +150,7 @@ def main():
         try:
             with open(vpath) as f:
                 vj = json.load(f)
-            members = [k.split("<")[0] if k.startswith("k_scan_tiles") else k for k in dom_kernel.split("+")]
+            members = dom_kernel.split("+")
             members = [next((n for n in vj if n.split("<")[0] == m), m) for m in members]
             if all(m in vj for m in members):
                 n_valu = sum(vj[m]["valu"] for m in members)

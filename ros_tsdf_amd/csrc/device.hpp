@@ -54,7 +54,8 @@ struct EngCtr {
   int32_t fail_n;     // entries inserted in this pass that got no heap block
   int32_t sel_n;      // selected blocks (publish / export)
   int32_t freed_n;    // blocks freed by the current gc
-  int32_t pad[3];
+  int32_t new_n;      // entries inserted in this pass, waiting for heap blocks
+  int32_t pad[2];
   int32_t fold_n[8];  // touched blocks per fold size class
   unsigned long long stats[ST_COUNT];
 };
@@ -100,6 +101,8 @@ struct Dev {
   uint8_t* touched_map;      // [entries] map slot of each touched block
   uint32_t* tile_sums;       // [entries / 1024 + 2]
   uint32_t* fail_list;       // [entries] inserted entries without a heap block (removed after binning)
+  uint32_t* new_list;        // [entries] entries inserted in this pass (heap blocks assigned after binning)
+  uint8_t* new_map;          // [entries] their map slots
   int4* fold_desc;           // per size class c, from fold_base[c]: {record base, records, heap block, map}
   unsigned long long* part;  // [ST_COUNT][kPartSlots] statistics partial sums
   int64_t fold_cap;

@@ -238,6 +238,7 @@ __global__ void __launch_bounds__(256) k_pass_begin(Dev D, uint32_t mask, PassIn
     if (t == 0) {
       D.ctr->touched_n = 0;
       D.ctr->fail_n = 0;
+      D.ctr->new_n = 0;
     }
     if (t < FC_COUNT) D.ctr->fold_n[t] = 0;
     if (inl.nscans >= 0) {
@@ -315,34 +316,6 @@ __device__ __forceinline__ int assign_heap(const Dev& D, bool active, uint32_t e
     D.vals[e] = D.heap_free[idx];
     return kAssignOk;
   }
-  D.vals[e] = -1;
-  D.fail_list[atomicAdd(&D.ctr->fail_n, 1)] = e;
-  return quota_fail ? kAssignQuotaFail : kAssignHeapFail;
-}
-
-// The same for one lane alone (entries beyond a workgroup's new-entry list).
-__device__ __forceinline__ int assign_heap_one(const Dev& D, uint32_t e, int ms) {
-  const int64_t quota = D.maps[ms].block_quota;
-  const int held = atomicAdd(&D.mctr[ms].blocks, 1);
-  bool ok = true;
-  bool quota_fail = false;
-  if (quota > 0 && held >= quota) {
-    ok = false;
-    quota_fail = true;
-  }
-  int idx = -1;
-  if (ok) {
-    idx = atomicSub(&D.ctr->heap_top, 1) - 1;
-    if (idx < 0) {
-      atomicAdd(&D.ctr->heap_top, 1);
-      ok = false;
-    }
-  }
-  if (ok) {
-    D.vals[e] = D.heap_free[idx];
-    return kAssignOk;
-  }
-  atomicSub(&D.mctr[ms].blocks, 1);
   D.vals[e] = -1;
   D.fail_list[atomicAdd(&D.ctr->fail_n, 1)] = e;
   return quota_fail ? kAssignQuotaFail : kAssignHeapFail;
@@ -445,32 +418,29 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
   __shared__ uint32_t trec[kBinSlots];
   __shared__ uint32_t tbase[kBinSlots];
   __shared__ uint32_t lnew[kBinSlots];
-  __shared__ int lnew_map[kBinSlots];
+  __shared__ uint8_t lnew_map[kBinSlots];
   __shared__ uint32_t ltouch[kBinSlots];
   __shared__ uint8_t ltouch_map[kBinSlots];
   __shared__ uint16_t locc[kBinSlots];
-  __shared__ int n_new, n_touch, g_touch, n_occ;
+  __shared__ int n_new, n_touch, g_touch, g_new, n_occ;
   const PassHdr hdr = *D.hdr;
   const int P = hdr.total;
-  uint32_t st[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // per-thread counts
-  const int which[11] = {ST_VALID, ST_SKIPPED, ST_BLOCK_CELLS, ST_UPDATES, ST_TRUNC_VOX, ST_TRUNC_BLK,
-                         ST_INSERT_FAIL, ST_UPD_DROPPED, ST_NEW_BLOCKS, ST_HEAP_FAIL, ST_QUOTA_FAIL};
-  // heap failures include quota failures, which are also counted apart
-  auto count_assign = [&](int r) {
-    st[8] += r == kAssignOk ? 1 : 0;
-    st[9] += (r == kAssignHeapFail || r == kAssignQuotaFail) ? 1 : 0;
-    st[10] += r == kAssignQuotaFail ? 1 : 0;
-  };
-  // a new entry joins the workgroup's list (heap blocks are handed out for
-  // the whole list at the end of the workgroup's points), or gets its block
-  // at once when the list is full
+  uint32_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per-thread counts
+  const int which[8] = {ST_VALID, ST_SKIPPED, ST_BLOCK_CELLS, ST_UPDATES, ST_TRUNC_VOX, ST_TRUNC_BLK,
+                        ST_INSERT_FAIL, ST_UPD_DROPPED};
+  // a new entry joins the workgroup's list, which goes to the pass's
+  // new-entry list with one atomic at the end of the workgroup's points
+  // (heap blocks are assigned by k_pass_scan), or joins the pass's list at
+  // once when the workgroup's list is full
   auto note_new = [&](int e, int ms) {
     const int at = atomicAdd(&n_new, 1);
     if (at < kBinSlots) {
       lnew[at] = (uint32_t)e;
       lnew_map[at] = ms;
     } else {
-      count_assign(assign_heap_one(D, (uint32_t)e, ms));
+      const int g = atomicAdd(&D.ctr->new_n, 1);
+      D.new_list[g] = (uint32_t)e;
+      D.new_map[g] = (uint8_t)ms;
     }
   };
   // global lookup of table slot x (phase B)
@@ -673,20 +643,19 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
       resolve_slot(locc[y], e);
     }
     __syncthreads();
-    if (threadIdx.x == 0) g_touch = n_touch ? atomicAdd(&D.ctr->touched_n, n_touch) : 0;
-    {
-      const int nn = min(n_new, kBinSlots);
-      for (int x0 = 0; x0 < nn; x0 += blockDim.x) {
-        const int x = x0 + threadIdx.x;
-        const bool active = x < nn;
-        count_assign(assign_heap(D, active, active ? lnew[x] : 0u, active ? lnew_map[x] : -1));
-      }
+    const int nn = min(n_new, kBinSlots);
+    if (threadIdx.x == 0) {
+      g_touch = n_touch ? atomicAdd(&D.ctr->touched_n, n_touch) : 0;
+      g_new = nn ? atomicAdd(&D.ctr->new_n, nn) : 0;
     }
     __syncthreads();
     for (int x = threadIdx.x; x < n_touch; x += blockDim.x) {
-      const uint32_t e = ltouch[x];
-      D.touched[g_touch + x] = e;
+      D.touched[g_touch + x] = ltouch[x];
       D.touched_map[g_touch + x] = ltouch_map[x];
+    }
+    for (int x = threadIdx.x; x < nn; x += blockDim.x) {
+      D.new_list[g_new + x] = lnew[x];
+      D.new_map[g_new + x] = lnew_map[x];
     }
     // ---- phase C: final (entry, record offset) of every run ----
     if (i < P) {
@@ -844,6 +813,43 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int32_t* n_ptr, Count 
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_sums[ntiles];
+}
+
+// First kernel after binning: per-tile record totals of the touched blocks
+// (as k_scan_tiles with CountTouchedRecords), and heap blocks for the pass's
+// new entries, 256 per workgroup and chunk (the chunks are taken from the
+// last workgroup down, so the tile workgroups get them last).  Heap blocks
+// come off the free stack with one pop, and one per-map count, per wave.
+__global__ void __launch_bounds__(256) k_pass_scan(Dev D) {
+  __shared__ uint32_t sh[kScanSh];
+  const int n = ld_acq(&D.ctr->touched_n);
+  const int ntiles = (n + kScanTile - 1) / kScanTile;
+  const CountTouchedRecords cnt{D.touched, D.blk_cnt};
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanTile / 256; ++k) {
+      const int i = t * kScanTile + k * 256 + threadIdx.x;
+      if (i < n) v += cnt(i);
+    }
+    uint32_t tot;
+    block_scan_excl(v, sh, tot);
+    if (threadIdx.x == 0) D.tile_sums[t] = tot;
+  }
+  // heap failures include quota failures, which are also counted apart
+  uint32_t st[3] = {0, 0, 0};
+  const int which[3] = {ST_NEW_BLOCKS, ST_HEAP_FAIL, ST_QUOTA_FAIL};
+  const int nn = ld_acq(&D.ctr->new_n);
+  const int nchunks = (nn + 255) / 256;
+  for (int c = gridDim.x - 1 - blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int x = c * 256 + threadIdx.x;
+    const bool active = x < nn;
+    const int r = assign_heap(D, active, active ? D.new_list[x] : 0u, active ? (int)D.new_map[x] : -1);
+    st[0] += r == kAssignOk ? 1 : 0;
+    st[1] += (r == kAssignHeapFail || r == kAssignQuotaFail) ? 1 : 0;
+    st[2] += r == kAssignQuotaFail ? 1 : 0;
+  }
+  flush_stats(D, which, st);
 }
 
 // ---- records ---------------------------------------------------------------------------

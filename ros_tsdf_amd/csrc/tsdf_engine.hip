@@ -137,6 +137,8 @@ struct tsdf_engine {
   uint32_t* touched = nullptr;
   uint32_t* tile_sums = nullptr;
   uint32_t* fail_list = nullptr;
+  uint32_t* new_list = nullptr;
+  uint8_t* new_map = nullptr;
   int4* fold_desc = nullptr;
   unsigned long long* part = nullptr;
   int64_t fold_cap = 0;
@@ -197,6 +199,8 @@ struct tsdf_engine {
     D.touched = touched;
     D.tile_sums = tile_sums;
     D.fail_list = fail_list;
+    D.new_list = new_list;
+    D.new_map = new_map;
     D.fold_desc = fold_desc;
     D.part = part;
     D.fold_cap = fold_cap;
@@ -398,14 +402,13 @@ int enqueue_pass(tsdf_engine* E, int64_t P, int tid, const PassInline& inl) {
   const int tb = L.traverse_block;
   hipLaunchKernelGGL(k_bin, dim3(grid_for(P, tb, L.traverse_grid_cap)), dim3(tb), 0, st, D);
   CHK(timing_mark(E, tid, 1));
-  // (heap assignment runs inside k_bin: the assign stage is empty)
+  // (heap assignment runs in k_pass_scan, inside the scan stage: the assign stage is empty)
   CHK(timing_mark(E, tid, 2));
   const int64_t tmax = std::min<int64_t>(E->entries, P * E->stride_cap);
   // tiles of touched blocks are taken grid-stride by at most 256 workgroups
   // (the touched count is only known on the device; most passes have tens)
   const int sg = grid_for(tmax, kScanTile, 256);
-  CountTouchedRecords ct{E->touched, E->blk_cnt};
-  hipLaunchKernelGGL(k_scan_tiles<CountTouchedRecords>, dim3(sg), dim3(256), 0, st, &E->ctr->touched_n, ct, E->tile_sums);
+  hipLaunchKernelGGL(k_pass_scan, dim3(sg), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k_seg_apply, dim3(sg), dim3(256), 0, st, D);
   CHK(timing_mark(E, tid, 3));
   hipLaunchKernelGGL(k_emit, dim3(grid_for(P, 256, 65536)), dim3(256), 0, st, D);
@@ -734,6 +737,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
       (r = dev_alloc(&E->desc_dev, 1)) ||
       (r = dev_alloc(&E->touched, entries)) ||
       (r = dev_alloc(&E->tile_sums, entries / kScanTile + 2)) || (r = dev_alloc(&E->fail_list, entries)) ||
+      (r = dev_alloc(&E->new_list, entries)) || (r = dev_alloc(&E->new_map, entries)) ||
       (r = dev_alloc(&E->touched_map, entries)) || (r = dev_alloc(&E->part, ST_COUNT * kPartSlots)) ||
       (r = dev_alloc(&E->sel, entries)) || (r = dev_alloc(&E->pub_cnt, entries + 1)) ||
       (r = dev_alloc(&E->pub_off, entries + 1)))
@@ -791,7 +795,8 @@ int tsdf_engine_destroy(tsdf_engine* E) {
     }
   void* ptrs[] = {E->maps_dev, E->mctr, E->ctr, E->keys, E->vals, E->blk_cnt, E->blk_base, E->next, E->ovf_stack,
                   E->heap, E->heap_free, E->freed, E->desc_dev, E->pair_e, E->pair_r, E->pair_n,
-                  E->seg, E->touched, E->tile_sums, E->fail_list, E->fold_desc, E->part, E->touched_map, E->sel,
+                  E->seg, E->touched, E->tile_sums, E->fail_list, E->new_list, E->new_map, E->fold_desc, E->part,
+                  E->touched_map, E->sel,
                   E->pub_cnt, E->pub_off, E->pub_buf};
   for (void* p : ptrs)
     if (p) hipFree(p);
