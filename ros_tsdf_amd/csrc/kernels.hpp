@@ -331,6 +331,7 @@ __device__ __forceinline__ int assign_heap(const Dev& D, bool active, uint32_t e
 // its record offset in the block, stored with its entry for k_emit.  A full
 // LDS table falls back to per-run global atomics.
 constexpr int kBinSlots = 1024;
+constexpr int kBlkReg = 4;  // block-walk cells a lane keeps in registers (MAX_BLK of the configs)
 constexpr uint32_t kPairFail = 0xFFFFFFFFu;
 constexpr uint32_t kPairDirect = 0x80000000u;
 
@@ -488,6 +489,17 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
     ScanDev sd;
     MapDev m;
     float p[3], s[3], a[3], b[3];
+    // the block walk's first kBlkReg cells and their table slots (-1: not in
+    // the table), so that the voxel walk's runs find their slots without
+    // probing the table
+    int32_t bk0[kBlkReg], bk1[kBlkReg], bk2[kBlkReg];
+    int bsl[kBlkReg];
+#pragma unroll
+    for (int u = 0; u < kBlkReg; ++u) {
+      bk0[u] = bk1[u] = bk2[u] = 0;
+      bsl[u] = -1;
+    }
+    int nbc = 0;
     if (i < P) {
       sd = D.scans[find_scan(D.scans, hdr.nscans, i)];
       m = D.maps[sd.map];
@@ -507,12 +519,22 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
         bool tb = false;
         const int nb = walk_cells(a, b, m.BL, m.sc_b, m.isc_b, m.half_BL, m.max_blk, tb, [&](int32_t k0, int32_t k1, int32_t k2) {
           const int32_t kb[3] = {k0, k1, k2};
-          if (lds_find_or_add(tkey, lds_block_key(sd.map, kb)) < 0) {
+          const int slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb));
+          if (slot < 0) {
             bool ins;
             const int e = find_or_insert(D, m, pack_key(k0, k1, k2), ins);
             if (e < 0) st[6]++;
             else if (ins) note_new(e, m.slot);
           }
+#pragma unroll
+          for (int u = 0; u < kBlkReg; ++u)
+            if (u == nbc) {
+              bk0[u] = k0;
+              bk1[u] = k1;
+              bk2[u] = k2;
+              bsl[u] = slot;
+            }
+          ++nbc;
           return true;
         });
         st[2] += nb;
@@ -549,8 +571,12 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
       // straight from the global counter.  A block the block walk did not
       // list joins the table's list here.
       auto close_run = [&](const int32_t kb[3], int rn) {
+        int slot = -1;
+#pragma unroll
+        for (int u = 0; u < kBlkReg; ++u)
+          if (bsl[u] >= 0 && bk0[u] == kb[0] && bk1[u] == kb[1] && bk2[u] == kb[2]) slot = bsl[u];
         bool added = false;
-        const int slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb), &added);
+        if (slot < 0) slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb), &added);
         if (slot >= 0) {
           if (added) locc[atomicAdd(&n_occ, 1)] = (uint16_t)slot;
           const uint32_t local = atomicAdd(&trec[slot], (uint32_t)rn);
