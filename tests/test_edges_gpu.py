@@ -5,7 +5,7 @@ points on exact cell and block boundaries, and coordinates near the validity lim
 
 Capacity cases: heap exhaustion, a chain limit that makes inserts fail, publish capacity truncation, the
 Voxel.msg layout, map admission against a heap budget, per-map block quotas, and a scan larger than the
-initial scratch.
+initial scratch.  Fold tiers: one block with exactly n records, n on both sides of every size-class bound.
 
 When blocks cannot be stored (heap, chain or quota exhaustion), the whole block's updates are dropped. Every
 block that is stored received all of its updates, so it matches the oracle's block bit-exactly on keys and
@@ -297,3 +297,54 @@ def test_packed_state_publish_and_gc_sequence(oracle_lib, fmt):
         assert np.array_equal(k_g, k_o) and np.array_equal(w_g != 0, w_o != 0)
         m.close()
         om.close()
+
+
+# Fold tiers at their bounds: one block gets exactly n records in one pass, for n on both sides of every
+# size-class bound (16/17, 32/33, 64/65 wave tiers; 128/129 wave vs workgroup; 1024/1025 workgroup tiles;
+# 4096/4097 counting tile vs sorted tiles; 8192/8193 one vs two sorted tiles).  Axis-aligned rays make the
+# count exact: with voxel 0.1 m and truncation 0.25 m, a point at x in (8.36, 8.44) updates voxels 81..86
+# (six records, all in block x = 10), and a point at x = 7.8 + 0.1 (r - 1) updates r voxels of block 10
+# (the rest of its run lands in block 9).  The points sit in up to 64 (y, z) voxel columns, one scan per
+# column (each scan's sensor on the column's axis), and the scans go through one batched pass.
+TIER_PARAMS = (0.1, 0.25, 1000.0, 2500.0, 22500.0)
+
+
+def tier_scans(n):
+    q, r = divmod(n, 6)
+    cols = 64
+    per = [q // cols + (1 if c < q % cols else 0) for c in range(cols)]
+    scans = []
+    for c in range(cols):
+        y = np.float32(0.1 * (c % 8) + 0.037)
+        z = np.float32(0.1 * (c // 8) + 0.061)
+        xs = [8.36 + 0.08 * (u + 0.5) / max(per[c], 1) for u in range(per[c])]
+        if c == 0 and r:
+            xs.append(7.8 + 0.1 * (r - 1))
+        if not xs:
+            continue
+        pts = as_points([[x, y, z] for x in xs])
+        scans.append((pts, np.array([0.0, y, z], np.float32)))
+    return scans
+
+
+@pytest.mark.parametrize("n", [16, 17, 32, 33, 64, 65, 128, 129, 1024, 1025, 4096, 4097, 8192, 8193])
+def test_fold_tiers_at_their_bounds(eng, oracle_lib, n):
+    scans = tier_scans(n)
+    pool = workloads.Pool([(p, s, 0) for p, s in scans])
+    buf = eng.buffer(pool.points.nbytes)
+    buf.upload(pool.points)
+    m = eng.map(TIER_PARAMS, buckets_log2=10)
+    eng.integrate_batch([m], [(0, pool.counts[i], pool.offsets[i], pool.sensors[i]) for i in range(len(pool))], buf)
+    eng.synchronize()
+    st = eng.stats()
+    om = oracle_lib.OracleMap(TIER_PARAMS)
+    for p, s in scans:
+        om.integrate(p, s)
+    # every point's run has six voxels; only the straddling point's run leaves block 10
+    assert st["voxel_updates"] == 6 * pool.points.shape[0]
+    k, _, w = m.export()
+    i10 = [j for j, kk in enumerate(k.tolist()) if kk == [10, 0, 0]]
+    assert len(i10) == 1
+    compare_maps(m, om, TIER_PARAMS[1], *tol(TIER_PARAMS))
+    m.close()
+    buf.free()
