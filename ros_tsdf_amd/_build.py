@@ -45,7 +45,7 @@ def build_engine() -> str:
 
 
 def build_package() -> str:
-    """The C++ drop-in (tsdf::TSDFHandler & co.) over the C ABI, plus its test driver."""
+    """The C++ drop-in (tsdf::TSDFHandler & co.) over the C ABI, plus its test program."""
     os.makedirs(LIB_DIR, exist_ok=True)
     tmp = PACKAGE_SO + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", INCLUDE,

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 def test_dropin_handler_matches_oracle(tmp_path, oracle_lib):
     if not os.path.exists(_build.HANDLER_TEST_BIN):
-        pytest.fail("drop-in test driver %s is not built" % _build.HANDLER_TEST_BIN)
+        pytest.fail("drop-in test program %s is not built" % _build.HANDLER_TEST_BIN)
     params = configs.SMALL_PARAMS
     scans = [(p, s) for _, p, s in synth.scans(configs.SMALL, stop=12)]
     inp = tmp_path / "in.bin"
