@@ -1361,6 +1361,103 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
   __syncthreads();
 }
 
+// One block beyond the counting tile, folded in voxel ranges: the voxel
+// counts of the whole block are taken first (a histogram over its records,
+// read from the segment), and a voxel belongs to range start / kRangeSpan,
+// where start is the number of the block's records in the voxels before it.
+// While no voxel holds more than kRangeSpan records, the ranges follow each
+// other without gaps and each holds fewer than 2 kRangeSpan <= kCountTile
+// records, so each range is sorted and folded like a counting tile, its
+// records taken from the segment once per range; voxel l's state is folded
+// by thread l in its range.  Returns false (with nothing folded) for a block
+// outside those bounds, which then takes the merge path.
+constexpr int kRangeSpan = kCountTile / 2;
+constexpr int kMaxRanges = 512;
+template <class State, int NT>
+__device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, unsigned long long* rin,
+                                                  unsigned long long* rout, uint32_t* cnt, uint32_t* start,
+                                                  uint32_t* rstart, uint32_t* sh, unsigned long long& uniq) {
+  static_assert(NT >= kVoxelsPerBlock, "a thread per voxel");
+  static_assert(2 * kRangeSpan <= kCountTile, "a range fits the counting tile");
+  __shared__ int too_big, nranges_sh;
+  const int c = d.y;
+  const unsigned long long* sg = D.seg + (uint32_t)d.x;
+  const int t = threadIdx.x;
+  if (t < kVoxelsPerBlock) cnt[t] = 0;
+  if (t == 0) too_big = c / kRangeSpan + 1 > kMaxRanges ? 1 : 0;
+  __syncthreads();
+  for (int x = t; x < c; x += NT) atomicAdd(&cnt[rec_voxel(sg[x])], 1u);
+  __syncthreads();
+  const uint32_t mine = t < kVoxelsPerBlock ? cnt[t] : 0u;
+  if (mine > (uint32_t)kRangeSpan) too_big = 1;
+  uint32_t tot;
+  const uint32_t ex = block_scan_excl(mine, sh, tot);  // (its barriers publish too_big)
+  if (too_big) return false;
+  // voxel t: its start, range, state.  The ranges are those of the voxels
+  // with records: 0 up to the range of the last such voxel, with no gap
+  // (consecutive voxels' starts differ by at most kRangeSpan).
+  const int rg = (int)(ex / kRangeSpan);
+  if (t < kVoxelsPerBlock) start[t] = ex;
+  if (t == 0) start[kVoxelsPerBlock] = tot;
+  if (mine != 0 && ex + mine == tot) nranges_sh = rg + 1;
+  const bool h = t < kVoxelsPerBlock && mine != 0;
+  __syncthreads();
+  // range r starts at the start of its first voxel: the voxel whose range
+  // differs from the previous voxel's
+  const int nranges = nranges_sh;
+  if (t < kVoxelsPerBlock && (t == 0 || (int)(start[t - 1] / kRangeSpan) != rg) && ex < tot) rstart[rg] = ex;
+  if (t == 0) rstart[nranges] = tot;
+  __syncthreads();
+  for (int r = 0; r < nranges; ++r) {
+    const uint32_t base = rstart[r];
+    const int nr = (int)(rstart[r + 1] - base);
+    if (t < kVoxelsPerBlock) cnt[t] = 0;
+    __syncthreads();
+    for (int x = t; x < c; x += NT) {
+      const unsigned long long rec = sg[x];
+      const int l = rec_voxel(rec);
+      const uint32_t sl = start[l];
+      if ((int)(sl / kRangeSpan) == r) rout[sl - base + atomicAdd(&cnt[l], 1u)] = rec;
+    }
+    __syncthreads();
+    // point order inside each voxel (as in fold_count_block)
+    for (int x = t; x < nr; x += NT) {
+      const unsigned long long rec = rout[x];
+      const int l = rec_voxel(rec);
+      const int b0 = (int)(start[l] - base), b1 = (int)(start[l + 1] - base);
+      int rank = 0;
+      int j = b0;
+      for (; j + 4 <= b1; j += 4) {
+        unsigned long long v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = rout[j + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) rank += v[u] < rec ? 1 : 0;
+      }
+      for (; j < b1; ++j) rank += rout[j] < rec ? 1 : 0;
+      rin[b0 + rank] = rec;
+    }
+    __syncthreads();
+    if (h && rg == r) {
+      // the voxel's state is read and written in its range's fold only
+      const MapDev& m = D.maps[d.w];
+      const Codec cd = make_codec(m);
+      float S, W;
+      State::load(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, S, W);
+      const int e = (int)(start[t + 1] - base);
+      for (int i = (int)(ex - base); i < e; ++i) {
+        float w, wd;
+        weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
+        fold_step(S, W, w, wd, m.maxW);
+      }
+      State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, S, W);
+      uniq++;
+    }
+    __syncthreads();
+  }
+  return true;
+}
+
 // One block beyond the counting tile: records sorted in LDS tiles of
 // kFoldCap written back in place, then every thread merges its voxel's runs
 // across the tiles in point order.  sk holds kFoldCap records.
@@ -1521,6 +1618,7 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
   __shared__ uint32_t cnt[kVoxelsPerBlock];
   __shared__ uint32_t start[kVoxelsPerBlock + 1];
   __shared__ uint32_t sh[kScanSh];
+  __shared__ uint32_t rstart[kMaxRanges + 1];
   unsigned long long uniq = 0, dropped = 0;
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG2]);
   const int4* desc = D.fold_desc + D.fold_base[FC_WG2];
@@ -1528,7 +1626,11 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
     fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[q], buf, buf + kCountTile, cnt, start, sh, uniq);
   const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
   const int4* hdesc = D.fold_desc + D.fold_base[FC_HUGE];
-  for (int q = blockIdx.x; q < n_huge; q += gridDim.x) fold_huge_block<State>(D, hdesc[q], buf, uniq, dropped);
+  for (int q = blockIdx.x; q < n_huge; q += gridDim.x) {
+    const int4 d = hdesc[q];
+    if (!fold_ranged_block<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, rstart, sh, uniq))
+      fold_huge_block<State>(D, d, buf, uniq, dropped);
+  }
   stat_add(D, ST_UNIQUE, uniq);
   stat_add(D, ST_UPD_DROPPED, dropped);
 }

@@ -301,7 +301,7 @@ def test_packed_state_publish_and_gc_sequence(oracle_lib, fmt):
 
 # Fold tiers at their bounds: one block gets exactly n records in one pass, for n on both sides of every
 # size-class bound (16/17, 32/33, 64/65 wave tiers; 128/129 wave vs workgroup; 1024/1025 workgroup tiles;
-# 4096/4097 counting tile vs sorted tiles; 8192/8193 one vs two sorted tiles).  Axis-aligned rays make the
+# 4096/4097 counting tile vs voxel ranges; 8192/8193 and 20000 several ranges; a voxel beyond a range).  Axis-aligned rays make the
 # count exact: with voxel 0.1 m and truncation 0.25 m, a point at x in (8.36, 8.44) updates voxels 81..86
 # (six records, all in block x = 10), and a point at x = 7.8 + 0.1 (r - 1) updates r voxels of block 10
 # (the rest of its run lands in block 9).  The points sit in up to 64 (y, z) voxel columns, one scan per
@@ -309,9 +309,8 @@ def test_packed_state_publish_and_gc_sequence(oracle_lib, fmt):
 TIER_PARAMS = (0.1, 0.25, 1000.0, 2500.0, 22500.0)
 
 
-def tier_scans(n):
+def tier_scans(n, cols=64):
     q, r = divmod(n, 6)
-    cols = 64
     per = [q // cols + (1 if c < q % cols else 0) for c in range(cols)]
     scans = []
     for c in range(cols):
@@ -327,9 +326,12 @@ def tier_scans(n):
     return scans
 
 
-@pytest.mark.parametrize("n", [16, 17, 32, 33, 64, 65, 128, 129, 1024, 1025, 4096, 4097, 8192, 8193])
-def test_fold_tiers_at_their_bounds(eng, oracle_lib, n):
-    scans = tier_scans(n)
+@pytest.mark.parametrize("n,cols", [(n, 64) for n in (16, 17, 32, 33, 64, 65, 128, 129, 1024, 1025, 4096, 4097,
+                                                      8192, 8193, 20000)] + [(12606, 1)])
+def test_fold_tiers_at_their_bounds(eng, oracle_lib, n, cols):
+    # blocks beyond 4096 records are folded in voxel ranges of about 2048 records; (12606, 1) puts 2101
+    # records in each of six voxels, past the range bound, which takes the merge path instead
+    scans = tier_scans(n, cols)
     pool = workloads.Pool([(p, s, 0) for p, s in scans])
     buf = eng.buffer(pool.points.nbytes)
     buf.upload(pool.points)
