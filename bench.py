@@ -268,6 +268,47 @@ def main():
         log("batched", result["batched"])
         for mm in maps:
             mm.close()
+        # the same comparison with full-size scans (131,072 points each): 8 maps x 4 scans per launch
+        fscans = [(bmaps_pts[mi][k], bmaps_sens[mi][k], mi) for b in range(3) for mi in range(configs.BATCH_MAPS)
+                  for k in range(4 * b, 4 * b + 4)]
+        fpool = make_pool(fscans)
+        fbuf = eng.buffer(fpool.points.nbytes)
+        fbuf.upload(fpool.points)
+        per = len(fscans) // 3
+        fdescs = [[(fpool.maps[j], fpool.counts[j], fpool.offsets[j], fpool.sensors[j]) for j in range(per * b, per * b + per)]
+                  for b in range(3)]
+        eng.set_launch(traverse_block=best["traverse_block"], fold_grid=best["fold_grid"])
+        maps = mk()
+        g = BatchGraph(eng, maps, max_points=max(sum(x[1] for x in d) for d in fdescs), max_scans=per)
+        g.launch(fdescs[0], fbuf)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        for d in fdescs[1:]:
+            g.launch(d, fbuf)
+            eng.synchronize()
+        tfb = time.perf_counter() - t0
+        fpts = sum(sum(x[1] for x in d) for d in fdescs[1:])
+        g.close()
+        for mm in maps:
+            mm.close()
+        maps = mk()
+        eng.set_launch()
+        tfp = 0.0
+        for i, d in enumerate(fdescs):
+            for (mi, n, off, s) in d:
+                eng.synchronize()
+                t0 = time.perf_counter()
+                maps[mi].integrate(fbuf, n, s, offset_points=off)
+                eng.synchronize()
+                if i > 0:
+                    tfp += time.perf_counter() - t0
+        result["batched_full_size"] = {"points_per_s": fpts / tfb, "plain_per_scan_points_per_s": fpts / tfp,
+                                       "speedup_vs_plain": tfp / tfb, "scans_per_launch": per,
+                                       "points_per_scan": fpts / (2 * per), "ms_per_launch": 1e3 * tfb / 2}
+        log("batched full-size", result["batched_full_size"])
+        for mm in maps:
+            mm.close()
+        fbuf.free()
         bbuf.free()
         eng.close()
 
