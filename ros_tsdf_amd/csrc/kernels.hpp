@@ -1,26 +1,27 @@
 // Kernels of the integration pipeline.  One pass integrates the scans of a
 // batch (a single scan on the per-scan path):
 //
-//   k_pass_begin     zero the pass counters and statistics partials
+//   k_pass_begin     zero the pass counters and statistics partials; store
+//                    the descriptors of a pass of up to 32 scans
 //   k_bin            per point: truncation segment, block DDA and voxel DDA;
 //                    find-or-insert every block in the map's spatial hash
 //                    (lock-free) once per workgroup; each run of voxels in
-//                    one block gets its record offset in that block; the
-//                    workgroup's new entries get heap blocks from the
-//                    free stack (one pop per wave)
-//   k_scan_tiles     per-tile totals of the touched blocks' record counts
+//                    one block gets its record offset in that block; new
+//                    entries join the pass's new-entry list
+//   k_pass_scan      per-tile totals of the touched blocks' record counts;
+//                    heap blocks for the new entries (one pop per wave)
 //   k_seg_apply      record segment offset and 16-byte fold descriptor of
 //                    every touched block, filed by size class
 //   k_emit           per point: second voxel-DDA walk writing each update
 //                    record (voxel, point, sdf) to its block's contiguous
 //                    record segment at the offset reserved in k_bin
-//   k_fold_a         blocks of 65..1024 records (a workgroup each: counting
+//   k_fold_a         blocks of 129..1024 records (a workgroup each: counting
 //                    sort by voxel in LDS, rank placement by point, fold),
-//                    then blocks of <= 64 / 32 / 16 records, 1 / 2 / 4 per
-//                    wave (bitonic sort across lanes with shuffles; each
-//                    voxel's first record folds the voxel's updates)
+//                    blocks of 65..128 (a wave each) and of <= 64 / 32 / 16
+//                    records, 1 / 2 / 4 per wave (bitonic sort across lanes;
+//                    each voxel's first record folds the voxel's updates)
 //   k_fold_b         blocks of 1025..4096 records (counting sort), then
-//                    larger ones (sorted LDS tiles merged per voxel)
+//                    larger ones in voxel ranges that fit the counting tile
 // Every voxel's state is read once and written once per pass, its updates
 // folded in point order.  Statistics are summed into spread partial slots
 // and reduced on demand (k_stats_reduce).
