@@ -13,6 +13,17 @@ def _fptr(a: np.ndarray):
     return a.ctypes.data_as(abi.fp)
 
 
+_F3 = C.c_float * 3
+
+
+def _vec3(v):
+    """A sensor origin as a float[3] argument (a copy, so any float-convertible 3-vector works)."""
+    a = np.ascontiguousarray(v, np.float32)
+    if a.size != 3:
+        raise ValueError("sensor origin must have 3 components, got %d" % a.size)
+    return _F3.from_buffer_copy(a)
+
+
 def params_struct(voxel_size, truncation_distance, max_weight, publish_distance_squared,
                   garbage_collect_distance_squared) -> abi.Params:
     return abi.Params(voxel_size, truncation_distance, max_weight, publish_distance_squared,
@@ -162,35 +173,34 @@ class Map:
         self.h = None
 
     def integrate(self, points: DeviceBuffer, n: int, sensor, offset_points: int = 0):
-        s = np.asarray(sensor, np.float32)
-        check(abi.load().tsdf_integrate(self.h, C.c_void_p(points.ptr + 16 * offset_points), n, _fptr(s)))
+        check(abi.load().tsdf_integrate(self.h, points.ptr + 16 * offset_points, n, _vec3(sensor)))
 
     def publish(self, sensor, capacity=None):
-        s = np.asarray(sensor, np.float32)
+        s = _vec3(sensor)
         n = C.c_int64()
         if capacity is None:
-            check(abi.load().tsdf_publish(self.h, _fptr(s), None, None, 0, C.byref(n)))
+            check(abi.load().tsdf_publish(self.h, s, None, None, 0, C.byref(n)))
             capacity = n.value
         pos = np.zeros((max(capacity, 1), 3), np.float32)
         data = np.zeros((max(capacity, 1), 2), np.float32)
-        check(abi.load().tsdf_publish(self.h, _fptr(s), _fptr(pos), _fptr(data), capacity, C.byref(n)))
+        check(abi.load().tsdf_publish(self.h, s, _fptr(pos), _fptr(data), capacity, C.byref(n)))
         k = min(n.value, capacity)
         return pos[:k], data[:k], n.value
 
     def publish_msg(self, sensor, capacity=None):
-        s = np.asarray(sensor, np.float32)
+        s = _vec3(sensor)
         n = C.c_int64()
         if capacity is None:
-            check(abi.load().tsdf_publish_msg(self.h, _fptr(s), None, 0, C.byref(n)))
+            check(abi.load().tsdf_publish_msg(self.h, s, None, 0, C.byref(n)))
             capacity = n.value
         msg = np.zeros((max(capacity, 1), 5), np.float32)
-        check(abi.load().tsdf_publish_msg(self.h, _fptr(s), _fptr(msg), capacity, C.byref(n)))
+        check(abi.load().tsdf_publish_msg(self.h, s, _fptr(msg), capacity, C.byref(n)))
         return msg[:min(n.value, capacity)], n.value
 
     def gc(self, sensor) -> int:
-        s = np.asarray(sensor, np.float32)
+        s = _vec3(sensor)
         r = C.c_int64()
-        check(abi.load().tsdf_gc(self.h, _fptr(s), C.byref(r)))
+        check(abi.load().tsdf_gc(self.h, s, C.byref(r)))
         return r.value
 
     def num_blocks(self) -> int:
@@ -224,13 +234,13 @@ class Map:
     def process_scan(self, points: np.ndarray, sensor, capacity=None):
         """The reference's processPointCloudAndUpdateVoxels sequence with host arrays."""
         pts = np.ascontiguousarray(points, np.float32)
-        s = np.asarray(sensor, np.float32)
+        s = _vec3(sensor)
         n = C.c_int64()
         if capacity is None:
             capacity = 1 << 20
         pos = np.zeros((capacity, 3), np.float32)
         data = np.zeros((capacity, 2), np.float32)
-        check(abi.load().tsdf_process_scan(self.h, _fptr(pts), pts.shape[0], _fptr(s), _fptr(pos), _fptr(data),
+        check(abi.load().tsdf_process_scan(self.h, _fptr(pts), pts.shape[0], s, _fptr(pos), _fptr(data),
                                            capacity, C.byref(n)))
         k = min(n.value, capacity)
         return pos[:k], data[:k], n.value
