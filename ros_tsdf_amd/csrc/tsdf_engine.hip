@@ -152,6 +152,9 @@ struct tsdf_engine {
   // publish output (device)
   float* pub_buf = nullptr;
   int64_t pub_buf_floats = 0;
+  // host points of tsdf_process_scan (device copy, grown on demand)
+  float* in_buf = nullptr;
+  int64_t in_buf_floats = 0;
 
   // maps
   tsdf_map* map_by_slot[kMaxMaps] = {};
@@ -533,7 +536,7 @@ int publish_impl(tsdf_map* m, const float sensor[3], float* pos, float* data, in
   HIPCHK(hipGetLastError());
   CHK(read_ctr(E));
   const int64_t total = (int64_t)E->ctr_host->stats[ST_PUBLISHED];
-  const int fpv = layout == 0 ? 5 : 5;
+  constexpr int fpv = 5;  // floats per voxel in either layout
   if (total * fpv > E->pub_buf_floats) {
     hipFree(E->pub_buf);
     E->pub_buf = nullptr;
@@ -797,7 +800,7 @@ int tsdf_engine_destroy(tsdf_engine* E) {
                   E->heap, E->heap_free, E->freed, E->desc_dev, E->pair_e, E->pair_r, E->pair_n,
                   E->seg, E->touched, E->tile_sums, E->fail_list, E->new_list, E->new_map, E->fold_desc, E->part,
                   E->touched_map, E->sel,
-                  E->pub_cnt, E->pub_off, E->pub_buf};
+                  E->pub_cnt, E->pub_off, E->pub_buf, E->in_buf};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (E->ctr_host) hipHostFree(E->ctr_host);
@@ -1009,16 +1012,27 @@ int tsdf_process_scan(tsdf_map* m, const float* points_host, int64_t n, const fl
   if (n < 0 || n > kMaxPassPoints) return fail(TSDF_ERR_CAPACITY, "point count %lld out of range", (long long)n);
   tsdf_engine* E = m->eng;
   CHK(set_device(E));
-  float* dpts = nullptr;
-  CHK(dev_alloc(&dpts, std::max<int64_t>(n, 1) * 4));
+  // the points go through an engine buffer kept between calls (the stream is
+  // idle when it grows: every call ends synchronised)
+  const int64_t want = std::max<int64_t>(n, 1) * 4;
+  if (want > E->in_buf_floats) {
+    HIPCHK(hipStreamSynchronize(E->stream));
+    (void)hipFree(E->in_buf);
+    E->in_buf = nullptr;
+    E->in_buf_floats = 0;
+    const int64_t grow = std::max<int64_t>(want, (int64_t)1 << 20);
+    CHK(dev_alloc(&E->in_buf, grow));
+    E->in_buf_floats = grow;
+  }
+  float* dpts = E->in_buf;
   int r = TSDF_OK;
   hipError_t e = hipMemcpyAsync(dpts, points_host, sizeof(float) * 4 * n, hipMemcpyHostToDevice, E->stream);
   if (e != hipSuccess) r = fail(TSDF_ERR_HIP, "H2D copy failed: %s", hipGetErrorString(e));
   if (!r) r = tsdf_integrate(m, dpts, n, sensor);
   if (!r) r = tsdf_publish(m, sensor, pos, data, capacity, count);
   if (!r) r = tsdf_gc(m, sensor, nullptr);
-  hipStreamSynchronize(E->stream);
-  hipFree(dpts);
+  e = hipStreamSynchronize(E->stream);
+  if (!r && e != hipSuccess) r = fail(TSDF_ERR_HIP, "stream synchronisation failed: %s", hipGetErrorString(e));
   return r;
 }
 

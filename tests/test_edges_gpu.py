@@ -299,6 +299,33 @@ def test_packed_state_publish_and_gc_sequence(oracle_lib, fmt):
         om.close()
 
 
+def test_process_scan_input_buffer_grows():
+    """processPointCloudAndUpdateVoxels with host points of growing and shrinking size (1000, then 300000,
+    past the engine's first input buffer, then 500): every call gives exactly the published set of the
+    explicit sequence upload -> integrate -> publish -> gc on a twin map."""
+    from ros_tsdf_amd import Engine
+    with Engine(heap_blocks=1 << 17, total_buckets=1 << 17, max_batch_points=1 << 19) as e:
+        a = e.map(D, buckets_log2=14)
+        b = e.map(D, buckets_log2=14)
+        buf = e.buffer(300000 * 16)
+        for k, n in enumerate((1000, 300000, 500)):
+            pts, s = dense_scan(k, n=n)
+            assert pts.shape[0] == n
+            pos_a, data_a, n_a = a.process_scan(pts, s, capacity=1 << 22)
+            buf.upload(np.ascontiguousarray(pts, np.float32))
+            b.integrate(buf, n, s)
+            pos_b, data_b, n_b = b.publish(s)
+            b.gc(s)
+            assert n_a == n_b > 0
+            oa, ob = np.lexsort(pos_a.T[::-1]), np.lexsort(pos_b.T[::-1])
+            assert np.array_equal(pos_a[oa].view(np.uint32), pos_b[ob].view(np.uint32))
+            assert np.array_equal(data_a[oa].view(np.uint32), data_b[ob].view(np.uint32))
+        assert a.num_blocks() == b.num_blocks()
+        buf.free()
+        a.close()
+        b.close()
+
+
 # Fold tiers at their bounds: one block gets exactly n records in one pass, for n on both sides of every
 # size-class bound (16/17, 32/33, 64/65 wave tiers; 128/129 wave vs workgroup; 1024/1025 workgroup tiles;
 # 4096/4097 counting tile vs voxel ranges; 8192/8193 and 20000 several ranges; a voxel beyond a range).  Axis-aligned rays make the
