@@ -18,6 +18,7 @@ ENGINE_SO = os.path.join(LIB_DIR, "libtsdf_hip.so")
 PACKAGE_SO = os.path.join(LIB_DIR, "libtsdf_package.so")
 HANDLER_TEST_BIN = os.path.join(LIB_DIR, "handler_dropin_test")
 
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: no FMA contraction (bit-exact FP32 geometry);
@@ -53,9 +54,10 @@ def build_package() -> str:
           "-L", LIB_DIR, "-ltsdf_hip", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, PACKAGE_SO)
     tmpb = HANDLER_TEST_BIN + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-I", INCLUDE, "-o", tmpb,
-          os.path.join(ROOT, "tests", "cpp", "handler_dropin_test.cpp"),
-          "-L", LIB_DIR, "-ltsdf_package", "-ltsdf_hip", "-Wl,-rpath,$ORIGIN"])
+    _run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", INCLUDE, "-I", os.path.join(ROCM, "include"),
+          "-o", tmpb, os.path.join(ROOT, "tests", "cpp", "handler_dropin_test.cpp"),
+          "-L", LIB_DIR, "-ltsdf_package", "-ltsdf_hip", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN"])
     os.replace(tmpb, HANDLER_TEST_BIN)
     return PACKAGE_SO
 

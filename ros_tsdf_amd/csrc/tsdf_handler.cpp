@@ -1,13 +1,15 @@
 // tsdf::TSDFHandler / tsdf::TSDFContainer / tsdf::initGlobalVars over the C
 // ABI of the HIP engine: the reference's host API (tsdf_handler.cuh:41-81,
 // tsdf_container.cuh:112-133) for callers such as the ROS2 node.
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
-#include <vector>
 
 #include "tsdf/tsdf_handler.hpp"
 #include "tsdf_hip.h"
+
+static_assert(offsetof(tsdf::Voxel, sdf) == 0 && offsetof(tsdf::Voxel, weight) == 4, "Voxel is {float sdf, float weight}");
 
 namespace {
 
@@ -107,8 +109,11 @@ void TSDFHandler::processPointCloudAndUpdateVoxels(pcl::PointCloud<pcl::PointXYZ
   const float* pts = n > 0 ? reinterpret_cast<const float*>(&point_cloud->points[0]) : nullptr;
   int64_t count = 0;
   const int64_t cap = (publish_voxels_pos_h && publish_voxels_data_h) ? PUBLISH_VOXELS_MAX_SIZE : 0;
-  std::vector<float> pos((size_t)cap * 3), data((size_t)cap * 2);
-  int r = tsdf_process_scan(m, pts, n, sensor, pos.data(), data.data(), cap, &count);
+  // the caller's arrays are the engine's output layout (Vector3f: three packed
+  // floats, Voxel: {sdf, weight}), so the voxels are copied into them directly
+  float* pos = cap ? publish_voxels_pos_h[0].data() : nullptr;
+  float* data = cap ? &publish_voxels_data_h[0].sdf : nullptr;
+  int r = tsdf_process_scan(m, pts, n, sensor, pos, data, cap, &count);
   if (r != TSDF_OK) {
     report("processPointCloudAndUpdateVoxels");
     if (publish_voxels_size_h) *publish_voxels_size_h = 0;
@@ -116,11 +121,6 @@ void TSDFHandler::processPointCloudAndUpdateVoxels(pcl::PointCloud<pcl::PointXYZ
   }
   const int64_t k = count < cap ? count : cap;
   if (count > cap) std::fprintf(stderr, "tsdf: %lld voxels to publish, %lld fit the output arrays\n", (long long)count, (long long)cap);
-  for (int64_t i = 0; i < k; ++i) {
-    publish_voxels_pos_h[i] = Vector3f(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-    publish_voxels_data_h[i].sdf = data[2 * i];
-    publish_voxels_data_h[i].weight = data[2 * i + 1];
-  }
   if (publish_voxels_size_h) *publish_voxels_size_h = (int)k;
 }
 
@@ -163,19 +163,15 @@ void TSDFHandler::retrievePublishableVoxels(Vector3f* lidar_position_d, Vector3f
     report("retrievePublishableVoxels");
     return;
   }
-  const int64_t cap = PUBLISH_VOXELS_MAX_SIZE;
-  std::vector<float> pos((size_t)cap * 3), data((size_t)cap * 2);
+  const int64_t cap = (publish_voxels_pos_h && publish_voxels_data_h) ? PUBLISH_VOXELS_MAX_SIZE : 0;
+  float* pos = cap ? publish_voxels_pos_h[0].data() : nullptr;
+  float* data = cap ? &publish_voxels_data_h[0].sdf : nullptr;
   int64_t count = 0;
-  if (tsdf_publish(m, s, pos.data(), data.data(), cap, &count) != TSDF_OK) {
+  if (tsdf_publish(m, s, pos, data, cap, &count) != TSDF_OK) {
     report("retrievePublishableVoxels");
     return;
   }
   const int64_t k = count < cap ? count : cap;
-  for (int64_t i = 0; i < k; ++i) {
-    publish_voxels_pos_h[i] = Vector3f(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-    publish_voxels_data_h[i].sdf = data[2 * i];
-    publish_voxels_data_h[i].weight = data[2 * i + 1];
-  }
   if (publish_voxels_size_h) *publish_voxels_size_h = (int)k;
 }
 

@@ -16,6 +16,24 @@ def _fptr(a: np.ndarray):
 _F3 = C.c_float * 3
 
 
+def _out_rows(a, cols):
+    """A caller's output array: float32, C-contiguous, cols floats per row."""
+    if not isinstance(a, np.ndarray) or a.dtype != np.float32 or not a.flags.c_contiguous or a.ndim != 2 \
+            or a.shape[1] != cols:
+        raise ValueError("output array must be a C-contiguous float32 array of shape (rows, %d)" % cols)
+    return a
+
+
+def _out_pair(out, capacity):
+    """(pos [c,3], data [c,2]) output arrays and the capacity they hold (c unless a smaller one is asked)."""
+    pos, data = _out_rows(out[0], 3), _out_rows(out[1], 2)
+    rows = min(pos.shape[0], data.shape[0])
+    capacity = rows if capacity is None else int(capacity)
+    if capacity > rows:
+        raise ValueError("capacity %d exceeds the output arrays (%d rows)" % (capacity, rows))
+    return pos, data, capacity
+
+
 def _vec3(v):
     """A sensor origin as a float[3] argument (a copy, so any float-convertible 3-vector works)."""
     a = np.ascontiguousarray(v, np.float32)
@@ -175,25 +193,39 @@ class Map:
     def integrate(self, points: DeviceBuffer, n: int, sensor, offset_points: int = 0):
         check(abi.load().tsdf_integrate(self.h, points.ptr + 16 * offset_points, n, _vec3(sensor)))
 
-    def publish(self, sensor, capacity=None):
+    def publish(self, sensor, capacity=None, out=None):
+        """Voxels to publish: (pos float32[k,3], data float32[k,2] {sdf, weight}, total).  k = min(total,
+        capacity).  out: (pos, data) float32 arrays of c rows to fill instead of new ones (capacity defaults
+        to c), which a caller keeps between scans as the reference's node keeps its publish arrays."""
         s = _vec3(sensor)
         n = C.c_int64()
-        if capacity is None:
-            check(abi.load().tsdf_publish(self.h, s, None, None, 0, C.byref(n)))
-            capacity = n.value
-        pos = np.zeros((max(capacity, 1), 3), np.float32)
-        data = np.zeros((max(capacity, 1), 2), np.float32)
+        if out is not None:
+            pos, data, capacity = _out_pair(out, capacity)
+        else:
+            if capacity is None:
+                check(abi.load().tsdf_publish(self.h, s, None, None, 0, C.byref(n)))
+                capacity = n.value
+            pos = np.zeros((max(capacity, 1), 3), np.float32)
+            data = np.zeros((max(capacity, 1), 2), np.float32)
         check(abi.load().tsdf_publish(self.h, s, _fptr(pos), _fptr(data), capacity, C.byref(n)))
         k = min(n.value, capacity)
         return pos[:k], data[:k], n.value
 
-    def publish_msg(self, sensor, capacity=None):
+    def publish_msg(self, sensor, capacity=None, out=None):
+        """Voxels to publish in the Voxel.msg layout {sdf, weight, x, y, z}: (msg float32[k,5], total).
+        out: a float32 array of c x 5 to fill instead of a new one."""
         s = _vec3(sensor)
         n = C.c_int64()
-        if capacity is None:
-            check(abi.load().tsdf_publish_msg(self.h, s, None, 0, C.byref(n)))
-            capacity = n.value
-        msg = np.zeros((max(capacity, 1), 5), np.float32)
+        if out is not None:
+            msg = _out_rows(out, 5)
+            capacity = msg.shape[0] if capacity is None else int(capacity)
+            if capacity > msg.shape[0]:
+                raise ValueError("capacity %d exceeds the output array" % capacity)
+        else:
+            if capacity is None:
+                check(abi.load().tsdf_publish_msg(self.h, s, None, 0, C.byref(n)))
+                capacity = n.value
+            msg = np.zeros((max(capacity, 1), 5), np.float32)
         check(abi.load().tsdf_publish_msg(self.h, s, _fptr(msg), capacity, C.byref(n)))
         return msg[:min(n.value, capacity)], n.value
 
@@ -231,15 +263,20 @@ class Map:
                                          C.byref(a), C.byref(b)))
         return int(a.value), int(b.value)
 
-    def process_scan(self, points: np.ndarray, sensor, capacity=None):
-        """The reference's processPointCloudAndUpdateVoxels sequence with host arrays."""
+    def process_scan(self, points: np.ndarray, sensor, capacity=None, out=None):
+        """The reference's processPointCloudAndUpdateVoxels sequence with host arrays: integrate the
+        float4 points, publish into (pos, data), gc.  out: (pos, data) arrays kept by the caller, as in
+        publish; otherwise new arrays of capacity rows (default 2^20)."""
         pts = np.ascontiguousarray(points, np.float32)
         s = _vec3(sensor)
         n = C.c_int64()
-        if capacity is None:
-            capacity = 1 << 20
-        pos = np.zeros((capacity, 3), np.float32)
-        data = np.zeros((capacity, 2), np.float32)
+        if out is not None:
+            pos, data, capacity = _out_pair(out, capacity)
+        else:
+            if capacity is None:
+                capacity = 1 << 20
+            pos = np.zeros((capacity, 3), np.float32)
+            data = np.zeros((capacity, 2), np.float32)
         check(abi.load().tsdf_process_scan(self.h, _fptr(pts), pts.shape[0], s, _fptr(pos), _fptr(data),
                                            capacity, C.byref(n)))
         k = min(n.value, capacity)

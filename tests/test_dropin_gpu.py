@@ -3,7 +3,10 @@
 `handler_dropin_test` links `libtsdf_package.so` like the node would. It calls initGlobalVars, then
 processPointCloudAndUpdateVoxels per scan, with caller-owned PUBLISH_VOXELS_MAX_SIZE arrays and a zeroed
 count. It writes each scan's published voxels. They must equal the oracle's integrate -> publish -> gc
-sequence (H:1003-1007): positions bit-exact as sets, values within tolerance.
+sequence (H:1003-1007): positions bit-exact as sets, values within tolerance.  The same holds when the
+program calls the reference's device-pointer stages itself ("members": allocateVoxelBlocksAndUpdateVoxels,
+retrievePublishableVoxels, garbageCollectDistantBlocks; "stages": getVoxelBlocks, allocateVoxelBlocks and
+getVoxelsAndUpdate in place of the first), on device copies of the scan's inputs.
 """
 import os
 import struct
@@ -18,7 +21,8 @@ from tests.helpers import compare_publish
 pytestmark = pytest.mark.gpu
 
 
-def test_dropin_handler_matches_oracle(tmp_path, oracle_lib):
+@pytest.mark.parametrize("mode", ["scan", "members", "stages"])
+def test_dropin_handler_matches_oracle(tmp_path, oracle_lib, mode):
     if not os.path.exists(_build.HANDLER_TEST_BIN):
         pytest.fail("drop-in test program %s is not built" % _build.HANDLER_TEST_BIN)
     params = configs.SMALL_PARAMS
@@ -31,8 +35,8 @@ def test_dropin_handler_matches_oracle(tmp_path, oracle_lib):
             f.write(np.ascontiguousarray(p, np.float32).tobytes())
     out = tmp_path / "out.bin"
     env = dict(os.environ, TSDF_HEAP_BLOCKS="65536")
-    r = subprocess.run([_build.HANDLER_TEST_BIN, str(inp), str(out)], env=env, capture_output=True, text=True,
-                       timeout=120)
+    r = subprocess.run([_build.HANDLER_TEST_BIN, str(inp), str(out), mode], env=env, capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     raw = out.read_bytes()
     om = oracle_lib.OracleMap(params)

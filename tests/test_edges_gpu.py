@@ -209,6 +209,21 @@ def test_publish_capacity_and_msg_layout(eng, oracle_lib):
     a = np.lexsort(as_msg.T[::-1])
     b = np.lexsort(msg.T[::-1])
     assert np.array_equal(as_msg[a].view(np.uint32), msg[b].view(np.uint32))
+    # caller-owned output arrays (kept between calls): the same voxels; a smaller capacity fills a prefix
+    out = (np.full((n + 7, 3), -1.0, np.float32), np.full((n + 7, 2), -1.0, np.float32))
+    def rows(*cols):  # a publish result as a sorted array of bit patterns (the voxel order is free)
+        r = np.concatenate(cols, 1).view(np.uint32)
+        return r[np.lexsort(r.T[::-1])]
+    pos_o, data_o, n_o = m.publish(s, out=out)
+    assert n_o == n and pos_o.shape[0] == n and np.shares_memory(pos_o, out[0])
+    assert np.array_equal(rows(pos_o, data_o), rows(pos, data))
+    assert (out[0][n:] == -1.0).all() and (out[1][n:] == -1.0).all()
+    pos_c, _, n_c = m.publish(s, capacity=cap, out=out)
+    assert n_c == n and pos_c.shape[0] == cap
+    with pytest.raises(ValueError):
+        m.publish(s, capacity=n + 8, out=out)
+    msg_o, nm_o = m.publish_msg(s, out=np.zeros((n, 5), np.float32))
+    assert nm_o == n and np.array_equal(rows(msg_o), rows(msg))
     m.close()
     om.close()
 
@@ -301,17 +316,19 @@ def test_packed_state_publish_and_gc_sequence(oracle_lib, fmt):
 
 def test_process_scan_input_buffer_grows():
     """processPointCloudAndUpdateVoxels with host points of growing and shrinking size (1000, then 300000,
-    past the engine's first input buffer, then 500): every call gives exactly the published set of the
-    explicit sequence upload -> integrate -> publish -> gc on a twin map."""
+    past the engine's first input buffer, then 500), publishing into output arrays kept between calls:
+    every call gives exactly the published set of the explicit sequence upload -> integrate -> publish ->
+    gc on a twin map (whose publish allocates its arrays)."""
     from ros_tsdf_amd import Engine
     with Engine(heap_blocks=1 << 17, total_buckets=1 << 17, max_batch_points=1 << 19) as e:
         a = e.map(D, buckets_log2=14)
         b = e.map(D, buckets_log2=14)
         buf = e.buffer(300000 * 16)
+        out = (np.zeros((1 << 22, 3), np.float32), np.zeros((1 << 22, 2), np.float32))  # kept between calls
         for k, n in enumerate((1000, 300000, 500)):
             pts, s = dense_scan(k, n=n)
             assert pts.shape[0] == n
-            pos_a, data_a, n_a = a.process_scan(pts, s, capacity=1 << 22)
+            pos_a, data_a, n_a = a.process_scan(pts, s, out=out)
             buf.upload(np.ascontiguousarray(pts, np.float32))
             b.integrate(buf, n, s)
             pos_b, data_b, n_b = b.publish(s)
