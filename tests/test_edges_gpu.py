@@ -4,8 +4,9 @@ Inputs: empty and ragged scans, invalid points (NaN, infinity, zero range, beyon
 points on exact cell and block boundaries, and coordinates near the validity limit.
 
 Capacity cases: heap exhaustion, a chain limit that makes inserts fail, publish capacity truncation, the
-Voxel.msg layout, map admission against a heap budget, per-map block quotas, and a scan larger than the
-initial scratch.  Fold tiers: one block with exactly n records, n on both sides of every size-class bound.
+Voxel.msg layout, map admission against a heap budget, per-map block quotas, a scan larger than the
+initial scratch, and more distinct blocks per binning workgroup than its LDS table holds.  Fold tiers: one
+block with exactly n records, n on both sides of every size-class bound.
 
 When blocks cannot be stored (heap, chain or quota exhaustion), the whole block's updates are dropped. Every
 block that is stored received all of its updates, so it matches the oracle's block bit-exactly on keys and
@@ -310,6 +311,41 @@ def test_packed_state_publish_and_gc_sequence(oracle_lib, fmt):
         k_g, _, w_g = m.export()
         k_o, _, w_o = om.export()
         assert np.array_equal(k_g, k_o) and np.array_equal(w_g != 0, w_o != 0)
+        m.close()
+        om.close()
+
+
+def test_workgroup_block_table_overflow(oracle_lib):
+    """More distinct blocks per 256-point workgroup than the binning kernel's LDS table holds (1024 slots):
+    2048 points scattered over a 1 km cube, a 1.2 m truncation band (up to 12 blocks and 96 voxels per point).
+    Every 256 consecutive points touch about 1440 distinct blocks (the oracle's walks check this), so the table
+    fills and the remaining blocks take the direct global path, more than 1024 of them are new (the
+    workgroup's new-entry list overflows to the pass list), and points have more runs than the emit kernel
+    keeps in registers.  The map equals the oracle's."""
+    import oracle
+    from ros_tsdf_amd import Engine
+    params = (0.1, 1.2, 1000.0, 2500.0, 22500.0)
+    rng = np.random.default_rng(11)
+    xyz = rng.uniform(-500, 500, size=(2048, 3)).astype(np.float32)
+    s = np.zeros(3, np.float32)
+    most_runs = 0
+    for c in range(0, xyz.shape[0], 256):
+        blocks = set()
+        for i in range(c, c + 256):
+            a, b = oracle.segment(xyz[i], s, params[1])
+            keys, _, _ = oracle.traverse(a, b, params[0], 96)
+            bk = [tuple(k) for k in (keys >> 3)]
+            blocks.update(bk)
+            most_runs = max(most_runs, 1 + sum(bk[j] != bk[j - 1] for j in range(1, len(bk))))
+        assert len(blocks) > 1024 + 256
+    assert most_runs > 6
+    with Engine(heap_blocks=1 << 15, total_buckets=1 << 16, max_batch_points=1 << 12) as e:
+        m, om, stats = integrate_both(e, oracle_lib, params, [(as_points(xyz), s)], buckets_log2=14)
+        g, o = stats[0]
+        assert g["insert_failures"] == 0 and g["heap_failures"] == 0 and g["updates_dropped"] == 0
+        assert g["voxel_updates"] == o["n_voxel_updates"] and g["unique_voxels"] == o["n_unique_voxels"]
+        assert g["new_blocks"] == m.num_blocks() > 8 * 1024
+        compare_maps(m, om, params[1], *tol(params))
         m.close()
         om.close()
 
