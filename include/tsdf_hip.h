@@ -147,8 +147,9 @@ int tsdf_map_import(tsdf_map* map, const int32_t* keys, const float* voxels, int
  * Replaces: TSDFHandler::allocateVoxelBlocksAndUpdateVoxels (H:1027-1060),
  * i.e. getVoxelBlocks (H:1071-1094), allocateVoxelBlocks (H:1103-1153) and
  * getVoxelsAndUpdate (H:1164-1182).  points_dev: device array of n
- * pcl::PointXYZ-layout float4 (x, y, z, pad).  Asynchronous on the engine
- * stream; the point buffer must stay valid until the work completes. */
+ * pcl::PointXYZ-layout float4 (x, y, z, pad), 16-byte aligned
+ * (TSDF_ERR_INVALID_ARG otherwise).  Asynchronous on the engine stream; the
+ * point buffer must stay valid until the work completes. */
 int tsdf_integrate(tsdf_map* map, const float* points_dev, int64_t n, const float sensor[3]);
 
 /* Replaces: TSDFHandler::retrievePublishableVoxels (H:1193-1224).  Writes

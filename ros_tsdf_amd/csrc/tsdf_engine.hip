@@ -461,6 +461,8 @@ int run_batch(tsdf_engine* E, tsdf_map* const* maps, int nmaps, const tsdf_scan_
     stride = std::max(stride, m->desc.max_vox);
   }
   if (P > kMaxPassPoints) return fail(TSDF_ERR_CAPACITY, "a pass holds at most %d points (got %lld)", kMaxPassPoints, (long long)P);
+  if (P > 0 && reinterpret_cast<uintptr_t>(points_dev) % alignof(float4) != 0)
+    return fail(TSDF_ERR_INVALID_ARG, "the point buffer must be 16-byte aligned (float4 rows)");
   CHK(set_device(E));
   CHK(ensure_scratch(E, std::max<int64_t>(P, 1), stride));
   // per-scan source offsets are folded into the point pointer: scans are

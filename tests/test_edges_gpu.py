@@ -103,6 +103,25 @@ def test_empty_scan_and_empty_batch(eng):
     buf.free()
 
 
+def test_misaligned_point_buffer_is_refused(eng):
+    """Points are float4 rows: a device pointer off a 16-byte boundary is refused before anything runs."""
+    import ctypes as C
+    pts, s = small_scan(0, n=64)
+    m = eng.map(P)
+    buf = eng.buffer(pts.nbytes + 16)
+    buf.upload(pts)
+    sensor = (C.c_float * 3)(*[float(x) for x in s])
+    for off in (4, 8, 12):
+        assert abi.load().tsdf_integrate(m.h, C.c_void_p(buf.ptr + off), pts.shape[0], sensor) == \
+            abi.TSDF_ERR_INVALID_ARG
+    assert m.num_blocks() == 0
+    m.integrate(buf, pts.shape[0], s)  # the aligned buffer integrates
+    eng.synchronize()
+    assert m.num_blocks() > 0
+    m.close()
+    buf.free()
+
+
 def test_invalid_points_skipped_like_the_oracle(eng, oracle_lib):
     pts, s = small_scan(0)
     lim = P[0] * 65536.0
