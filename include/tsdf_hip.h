@@ -301,8 +301,9 @@ typedef struct {
 } tsdf_cloud2_layout;
 /* cloud: the data block, in device memory when cloud_on_device is nonzero
  * (then asynchronous on the engine stream), else in host memory (copied
- * first; synchronous).  points_dev receives width * height float4 rows
- * (capacity rows at most: TSDF_ERR_CAPACITY beyond); *n_points their number.
+ * first; synchronous).  points_dev (16-byte aligned) receives width * height
+ * float4 rows (capacity rows at most: TSDF_ERR_CAPACITY beyond); *n_points
+ * their number.
  * A layout whose fields or rows reach past data_bytes, or fields of another
  * datatype, is refused with TSDF_ERR_INVALID_ARG. */
 int tsdf_cloud2_to_points(tsdf_engine* eng, const tsdf_cloud2_layout* layout, const void* cloud, int cloud_on_device,

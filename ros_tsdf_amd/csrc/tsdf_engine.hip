@@ -1307,6 +1307,8 @@ int tsdf_cloud2_to_points(tsdf_engine* E, const tsdf_cloud2_layout* L, const voi
   if (n == 0) return TSDF_OK;
   if (n > capacity) return fail(TSDF_ERR_CAPACITY, "cloud of %lld points, capacity %lld", (long long)n, (long long)capacity);
   if (!cloud || !points_dev) return fail(TSDF_ERR_INVALID_ARG, "null cloud or output");
+  if (reinterpret_cast<uintptr_t>(points_dev) % alignof(float4) != 0)
+    return fail(TSDF_ERR_INVALID_ARG, "the output must be 16-byte aligned (float4 rows)");
   // bytes read: up to the end of the last point of the last row
   const int64_t extent = (L->height - 1) * L->row_step + L->width * (int64_t)L->point_step;
   if (extent > L->data_bytes)

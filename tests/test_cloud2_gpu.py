@@ -120,6 +120,15 @@ def test_layout_errors(eng):
         with pytest.raises(abi.TsdfError) as ei:
             eng.cloud2_to_points(**args)
         assert ei.value.code == code, kw
+    # output rows off a 16-byte boundary (in a buffer a row longer, so the rows would still fit)
+    import ctypes as C
+    lay = abi.Cloud2Layout(width, height, row_step, buf.nbytes, 16, 0, 4, 8, abi.TSDF_FIELD_FLOAT32, 0)
+    n = C.c_int64()
+    wide = eng.buffer((width * height + 1) * 16)
+    assert abi.load().tsdf_cloud2_to_points(eng.h, C.byref(lay), C.c_void_p(buf.ctypes.data), 0,
+                                            C.c_void_p(wide.ptr + 8), width * height, C.byref(n)) == \
+        abi.TSDF_ERR_INVALID_ARG
+    wide.free()
     # the untouched layout still decodes
     assert eng.cloud2_to_points(buf, width, height, 16, row_step, (0, 4, 8), out) == width * height
 
