@@ -1192,13 +1192,14 @@ int tsdf_batch_graph_launch(tsdf_batch_graph* g, const tsdf_scan_desc* scans, in
   if (!g) return fail(TSDF_ERR_INVALID_ARG, "null graph");
   tsdf_engine* E = g->eng;
   if (nscans < 0 || nscans > g->max_scans) return fail(TSDF_ERR_CAPACITY, "too many scans for this graph");
+  if (nscans > 0 && !scans) return fail(TSDF_ERR_INVALID_ARG, "null scans");
   CHK(set_device(E));
   ScanDev sd[kMaxScansPerBatch];
   int64_t P = 0;
   for (int i = 0; i < nscans; ++i) {
     const tsdf_scan_desc& s = scans[i];
     if (s.map_index < 0 || s.map_index >= (int)g->maps.size()) return fail(TSDF_ERR_INVALID_ARG, "map_index out of range");
-    if (s.n_points < 0) return fail(TSDF_ERR_INVALID_ARG, "negative point count");
+    if (s.n_points < 0 || s.point_offset < 0) return fail(TSDF_ERR_INVALID_ARG, "scan %d: negative size/offset", i);
     if (s.point_offset != (nscans ? scans[0].point_offset : 0) + P)
       return fail(TSDF_ERR_INVALID_ARG, "scans of a batch must be contiguous and in order in the point buffer");
     for (int k = 0; k < 3; ++k)
@@ -1213,6 +1214,9 @@ int tsdf_batch_graph_launch(tsdf_batch_graph* g, const tsdf_scan_desc* scans, in
   }
   if (P > g->max_points) return fail(TSDF_ERR_CAPACITY, "batch of %lld points exceeds the graph's %lld", (long long)P,
                                      (long long)g->max_points);
+  if (P > 0 && !points_dev) return fail(TSDF_ERR_INVALID_ARG, "null point buffer");
+  if (P > 0 && reinterpret_cast<uintptr_t>(points_dev) % alignof(float4) != 0)
+    return fail(TSDF_ERR_INVALID_ARG, "the point buffer must be 16-byte aligned (float4 rows)");
   if (g->gen != E->scratch_gen) CHK(graph_capture(g));
   PassHdr hdr{};
   hdr.points = reinterpret_cast<const float4*>(points_dev) + (nscans ? scans[0].point_offset : 0);

@@ -100,6 +100,41 @@ def test_graph_replays_match_eager(eng):
     buf.free()
 
 
+def test_graph_launch_argument_errors(eng):
+    """A graph launch with a null or misaligned point buffer, a negative offset, scans out of order or a null
+    scan array is refused before anything is enqueued; the map stays empty and a valid launch still runs."""
+    import ctypes as C
+    from ros_tsdf_amd import BatchGraph
+    from ros_tsdf_amd.engine import _scan_array
+    scans = small_scans(1, 2)
+    pool = workloads.Pool(scans)
+    buf = eng.buffer(pool.points.nbytes + 16)
+    buf.upload(pool.points)
+    m = eng.map(P, buckets_log2=12)
+    g = BatchGraph(eng, [m], max_points=4000, max_scans=2)
+    d = [(0, pool.counts[i], pool.offsets[i], pool.sensors[i]) for i in range(len(pool))]
+    L = abi.load()
+
+    def launch(desc, ptr, n=None):
+        arr = _scan_array(desc)
+        return L.tsdf_batch_graph_launch(C.c_void_p(g.h), arr, len(desc) if n is None else n, C.c_void_p(ptr))
+
+    assert launch(d, None) == abi.TSDF_ERR_INVALID_ARG                        # no point buffer
+    assert launch(d, buf.ptr + 4) == abi.TSDF_ERR_INVALID_ARG                 # off a 16-byte boundary
+    neg = [(0, d[0][1], -1, d[0][3])]
+    assert launch(neg, buf.ptr) == abi.TSDF_ERR_INVALID_ARG                   # negative offset
+    assert launch(d[::-1], buf.ptr) == abi.TSDF_ERR_INVALID_ARG               # not contiguous and in order
+    assert L.tsdf_batch_graph_launch(C.c_void_p(g.h), None, 2, C.c_void_p(buf.ptr)) == abi.TSDF_ERR_INVALID_ARG
+    eng.synchronize()
+    assert m.num_blocks() == 0
+    g.launch(d, buf)
+    eng.synchronize()
+    assert m.num_blocks() > 0
+    g.close()
+    m.close()
+    buf.free()
+
+
 def test_serving_alone_vs_mixed(eng, oracle_lib):
     from ros_tsdf_amd import serve
     n_maps, per_map = 4, 6
