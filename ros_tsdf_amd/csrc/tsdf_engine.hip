@@ -1354,12 +1354,18 @@ int tsdf_serve_run(tsdf_engine* E, tsdf_map* const* maps, int32_t nmaps, const t
                    const float* points_dev, const tsdf_serve_config* cfg, double* latency_ms, int32_t* batch_of,
                    tsdf_serve_stats* stats) {
   if (!E || !maps || nmaps <= 0 || (nreq > 0 && !reqs) || !cfg) return fail(TSDF_ERR_INVALID_ARG, "null argument");
+  for (int i = 0; i < nmaps; ++i) {
+    CHK(check_map(maps[i]));
+    if (maps[i]->eng != E) return fail(TSDF_ERR_INVALID_ARG, "map %d belongs to another engine", i);
+  }
   const int max_scans = std::max(1, std::min<int>(cfg->max_batch_scans, kMaxScansPerBatch));
   const int max_inflight = std::max(1, std::min(cfg->max_inflight, kRing - 2));
   const int64_t max_pts = cfg->max_batch_points > 0 ? std::min<int64_t>(cfg->max_batch_points, kMaxPassPoints) : kMaxPassPoints;
   for (int i = 0; i < nreq; ++i) {
     if (reqs[i].map_index < 0 || reqs[i].map_index >= nmaps) return fail(TSDF_ERR_INVALID_ARG, "request %d: bad map", i);
     if (reqs[i].n_points < 0 || reqs[i].n_points > max_pts) return fail(TSDF_ERR_INVALID_ARG, "request %d: bad size", i);
+    if (reqs[i].point_offset < 0) return fail(TSDF_ERR_INVALID_ARG, "request %d: negative point offset", i);
+    if (reqs[i].n_points > 0 && !points_dev) return fail(TSDF_ERR_INVALID_ARG, "null point buffer");
     if (i > 0 && reqs[i].arrival_s < reqs[i - 1].arrival_s)
       return fail(TSDF_ERR_INVALID_ARG, "requests must be sorted by arrival time");
   }
@@ -1369,14 +1375,27 @@ int tsdf_serve_run(tsdf_engine* E, tsdf_map* const* maps, int32_t nmaps, const t
   // into a staging buffer with device-to-device copies.
   const int64_t stage_pts = max_pts;
   float* gather[kRing] = {};
-  for (int i = 0; i < max_inflight; ++i) CHK(dev_alloc(&gather[i], stage_pts * 4));
+  std::vector<hipEvent_t> evs(max_inflight, nullptr);
+  auto release = [&]() {
+    for (auto& e : evs)
+      if (e) hipEventDestroy(e);
+    for (int i = 0; i < max_inflight; ++i)
+      if (gather[i]) hipFree(gather[i]);
+  };
+  int rc = TSDF_OK;
+  for (int i = 0; i < max_inflight && rc == TSDF_OK; ++i) rc = dev_alloc(&gather[i], stage_pts * 4);
+  for (auto& e : evs)
+    if (rc == TSDF_OK && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      rc = fail(TSDF_ERR_HIP, "hipEventCreate failed");
+  if (rc != TSDF_OK) {
+    release();
+    return rc;
+  }
   struct Inflight {
     hipEvent_t done;
     std::vector<int> reqs;
     int buf;
   };
-  std::vector<hipEvent_t> evs(max_inflight);
-  for (auto& e : evs) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   std::vector<int> free_bufs;
   for (int i = 0; i < max_inflight; ++i) free_bufs.push_back(i);
   std::deque<Inflight> inflight;
@@ -1386,7 +1405,6 @@ int tsdf_serve_run(tsdf_engine* E, tsdf_map* const* maps, int32_t nmaps, const t
   const auto t0 = Clock::now();
   auto now_s = [&]() { return std::chrono::duration<double>(Clock::now() - t0).count(); };
   double first_arrival = nreq > 0 ? reqs[0].arrival_s : 0.0, last_done = 0.0;
-  int rc = TSDF_OK;
   std::vector<tsdf_scan_desc> sd;
   std::vector<tsdf_map*> bmaps(maps, maps + nmaps);
   while (done < nreq && rc == TSDF_OK) {
@@ -1453,7 +1471,11 @@ int tsdf_serve_run(tsdf_engine* E, tsdf_map* const* maps, int32_t nmaps, const t
       f.done = evs[b];
       f.reqs = taken;
       f.buf = b;
-      HIPCHK(hipEventRecord(f.done, E->stream));
+      const hipError_t re = hipEventRecord(f.done, E->stream);
+      if (re != hipSuccess) {
+        rc = fail(TSDF_ERR_HIP, "hipEventRecord failed: %s", hipGetErrorString(re));
+        break;
+      }
       inflight.push_back(f);
       batches++;
       points += P;
@@ -1468,8 +1490,7 @@ int tsdf_serve_run(tsdf_engine* E, tsdf_map* const* maps, int32_t nmaps, const t
     }
   }
   hipStreamSynchronize(E->stream);
-  for (auto& e : evs) hipEventDestroy(e);
-  for (int i = 0; i < max_inflight; ++i) hipFree(gather[i]);
+  release();
   if (stats) {
     stats->requests = nreq;
     stats->batches = batches;

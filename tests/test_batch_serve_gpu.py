@@ -135,6 +135,38 @@ def test_graph_launch_argument_errors(eng):
     buf.free()
 
 
+def test_serve_argument_errors(eng):
+    """The serving loop refuses a null point buffer, a negative point offset and a map of another engine
+    before it starts; the map stays empty and a valid run then serves every request."""
+    from ros_tsdf_amd import Engine, serve
+    scans = small_scans(1, 2)
+    pool = workloads.Pool(scans)
+    buf = eng.buffer(pool.points.nbytes)
+    buf.upload(pool.points)
+    m = eng.map(P, buckets_log2=12)
+    reqs = [(0.0, 0, pool.counts[i], pool.offsets[i], pool.sensors[i]) for i in range(len(pool))]
+
+    class NoBuffer:
+        ptr = None
+
+    def refused(maps, requests, points):
+        with pytest.raises(abi.TsdfError) as ei:
+            serve(eng, maps, requests, points)
+        return ei.value.code == abi.TSDF_ERR_INVALID_ARG
+
+    assert refused([m], reqs, NoBuffer())
+    assert refused([m], [(0.0, 0, reqs[0][2], -16, reqs[0][4])], buf)
+    with Engine(heap_blocks=1 << 12, total_buckets=1 << 12) as other:
+        om = other.map(P, buckets_log2=8)
+        assert refused([om], reqs, buf)
+        om.close()
+    assert m.num_blocks() == 0
+    lat, _, st = serve(eng, [m], reqs, buf)
+    assert st["requests"] == len(reqs) and (lat > 0).all() and m.num_blocks() > 0
+    m.close()
+    buf.free()
+
+
 def test_serving_alone_vs_mixed(eng, oracle_lib):
     from ros_tsdf_amd import serve
     n_maps, per_map = 4, 6
