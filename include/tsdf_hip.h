@@ -276,8 +276,9 @@ int tsdf_device_alloc(tsdf_engine* eng, int64_t bytes, void** out);
 int tsdf_device_free(tsdf_engine* eng, void* ptr);
 int tsdf_copy_to_device(tsdf_engine* eng, void* dst, const void* src, int64_t bytes);
 int tsdf_copy_to_host(tsdf_engine* eng, void* dst, const void* src, int64_t bytes);
-/* Device-to-device copy with a plain streaming kernel (bytes: multiple of 16);
- * synchronous.  Also serves as the known-size reference for HBM byte counters. */
+/* Device-to-device copy with a plain streaming kernel (bytes: multiple of 16,
+ * both pointers 16-byte aligned); synchronous.  Also serves as the
+ * known-size reference for HBM byte counters. */
 int tsdf_device_copy(tsdf_engine* eng, void* dst, const void* src, int64_t bytes);
 
 /* ---- input path ------------------------------------------------------------------

@@ -1278,6 +1278,8 @@ int tsdf_copy_to_host(tsdf_engine* E, void* dst, const void* src, int64_t bytes)
 int tsdf_device_copy(tsdf_engine* E, void* dst, const void* src, int64_t bytes) {
   if (!E || bytes < 0 || (bytes % 16) != 0 || (bytes > 0 && (!dst || !src)))
     return fail(TSDF_ERR_INVALID_ARG, "bad argument (bytes must be a multiple of 16)");
+  if ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) % alignof(float4) != 0)
+    return fail(TSDF_ERR_INVALID_ARG, "source and destination must be 16-byte aligned");
   CHK(set_device(E));
   const int64_t n = bytes / 16;
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256, 8192)), dim3(256), 0, E->stream, reinterpret_cast<const float4*>(src),

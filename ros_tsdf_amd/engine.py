@@ -65,13 +65,27 @@ class DeviceBuffer:
         b.upload(a)
         return b
 
+    def _span(self, offset: int, nbytes: int):
+        if self.ptr is None:
+            raise ValueError("the buffer was freed")
+        if offset < 0 or offset + nbytes > self.nbytes:
+            raise ValueError("bytes [%d, %d) outside the %d-byte buffer" % (offset, offset + nbytes, self.nbytes))
+
+    def check_points(self, offset_points: int, n: int):
+        """Raises unless float4 points [offset_points, offset_points + n) lie inside the buffer."""
+        if n < 0:
+            raise ValueError("negative point count")
+        self._span(16 * int(offset_points), 16 * int(n))
+
     def upload(self, a: np.ndarray, offset: int = 0):
         a = np.ascontiguousarray(a)
-        assert offset + a.nbytes <= self.nbytes
+        self._span(offset, a.nbytes)
         check(abi.load().tsdf_copy_to_device(self.engine.h, C.c_void_p(self.ptr + offset), a.ctypes.data, a.nbytes))
 
     def download(self, out: np.ndarray, offset: int = 0):
-        assert offset + out.nbytes <= self.nbytes
+        if not out.flags.c_contiguous:
+            raise ValueError("download target must be C-contiguous")
+        self._span(offset, out.nbytes)
         check(abi.load().tsdf_copy_to_host(self.engine.h, out.ctypes.data, C.c_void_p(self.ptr + offset), out.nbytes))
         return out
 
@@ -122,6 +136,8 @@ class Engine:
 
     def integrate_batch(self, maps, scans, points: DeviceBuffer):
         """scans: list of (map_index, n_points, point_offset, sensor)."""
+        for _, n, off, _ in scans:
+            points.check_points(off, n)
         arr = (abi.ScanDesc * len(scans))()
         for i, (mi, n, off, s) in enumerate(scans):
             arr[i].map_index = mi
@@ -191,6 +207,8 @@ class Map:
         self.h = None
 
     def integrate(self, points: DeviceBuffer, n: int, sensor, offset_points: int = 0):
+        if offset_points < 0 or n < 0 or 16 * (offset_points + n) > points.nbytes or points.ptr is None:
+            points.check_points(offset_points, n)  # raises with the reason
         check(abi.load().tsdf_integrate(self.h, points.ptr + 16 * offset_points, n, _vec3(sensor)))
 
     def publish(self, sensor, capacity=None, out=None):
@@ -315,6 +333,8 @@ class BatchGraph:
 
     def launch(self, scans, points: DeviceBuffer):
         """scans: (map_index, n_points, point_offset, sensor); contiguous and in order."""
+        for _, n, off, _ in scans:
+            points.check_points(off, n)
         arr = _scan_array(scans)
         check(abi.load().tsdf_batch_graph_launch(self.h, arr, len(scans), C.c_void_p(points.ptr)))
 
@@ -330,6 +350,9 @@ def serve(engine: Engine, maps, requests, points: DeviceBuffer, max_batch_scans=
     n_points, point_offset, sensor), sorted by arrival.  Returns
     (latency_ms array, batch index array, stats dict)."""
     n = len(requests)
+    if isinstance(points, DeviceBuffer):
+        for _, _, npts, off, _ in requests:
+            points.check_points(off, npts)
     arr = (abi.Request * max(n, 1))()
     for i, (t, mi, npts, off, s) in enumerate(requests):
         arr[i].arrival_s = float(t)

@@ -146,16 +146,17 @@ def test_serve_argument_errors(eng):
     m = eng.map(P, buckets_log2=12)
     reqs = [(0.0, 0, pool.counts[i], pool.offsets[i], pool.sensors[i]) for i in range(len(pool))]
 
-    class NoBuffer:
-        ptr = None
+    class Raw:  # a bare device pointer: the C ABI's own checks, without the Python buffer's range check
+        def __init__(self, ptr):
+            self.ptr = ptr
 
     def refused(maps, requests, points):
         with pytest.raises(abi.TsdfError) as ei:
             serve(eng, maps, requests, points)
         return ei.value.code == abi.TSDF_ERR_INVALID_ARG
 
-    assert refused([m], reqs, NoBuffer())
-    assert refused([m], [(0.0, 0, reqs[0][2], -16, reqs[0][4])], buf)
+    assert refused([m], reqs, Raw(None))
+    assert refused([m], [(0.0, 0, reqs[0][2], -16, reqs[0][4])], Raw(buf.ptr))
     with Engine(heap_blocks=1 << 12, total_buckets=1 << 12) as other:
         om = other.map(P, buckets_log2=8)
         assert refused([om], reqs, buf)

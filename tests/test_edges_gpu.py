@@ -122,6 +122,38 @@ def test_misaligned_point_buffer_is_refused(eng):
     buf.free()
 
 
+def test_point_ranges_outside_the_buffer_are_refused(eng):
+    """The Python API knows each device buffer's size: a scan, batch, graph or serving request whose points
+    reach past it (or start before it) is refused before any device work, as are out-of-range copies."""
+    from ros_tsdf_amd import BatchGraph, serve
+    pts, s = small_scan(0, n=64)
+    m = eng.map(P)
+    buf = eng.buffer(pts.nbytes)
+    buf.upload(pts)
+    bad = [(0, 65), (1, 64), (-1, 10)]  # (offset, n): one past the end, shifted, negative
+    for off, n in bad:
+        with pytest.raises(ValueError):
+            m.integrate(buf, n, s, offset_points=off)
+        with pytest.raises(ValueError):
+            eng.integrate_batch([m], [(0, n, off, s)], buf)
+        with pytest.raises(ValueError):
+            serve(eng, [m], [(0.0, 0, n, off, s)], buf)
+    g = BatchGraph(eng, [m], max_points=128, max_scans=1)
+    with pytest.raises(ValueError):
+        g.launch([(0, 65, 0, s)], buf)
+    g.close()
+    with pytest.raises(ValueError):
+        buf.upload(np.zeros(pts.nbytes + 1, np.uint8))
+    with pytest.raises(ValueError):
+        buf.download(np.zeros(16, np.uint8), offset=pts.nbytes - 8)
+    assert m.num_blocks() == 0
+    m.integrate(buf, 64, s)  # the whole buffer, exactly
+    eng.synchronize()
+    assert m.num_blocks() > 0
+    m.close()
+    buf.free()
+
+
 def test_invalid_points_skipped_like_the_oracle(eng, oracle_lib):
     pts, s = small_scan(0)
     lim = P[0] * 65536.0
