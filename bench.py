@@ -178,7 +178,7 @@ def main():
     roofline["copy_kernel_GBps"] = copy_gbps
     csrc.free()
     cdst.free()
-    plain_lat = None
+    plain_lat = plain_lat_pinned = None
     if extras:
         # ------------- full per-scan call latency (reference sequence) -------------
         m2 = eng.map(configs.DENSE_PARAMS, buckets_log2=17)
@@ -199,6 +199,20 @@ def main():
                      "scans": len(lat) - 2}
         m2.close()
         log("full-call latency", plain_lat)
+        # the same calls with page-locked host points and outputs (Engine.host_array), on a fresh map
+        m3 = eng.map(configs.DENSE_PARAMS, buckets_log2=17)
+        pin_out = (eng.host_array((1 << 22, 3)), eng.host_array((1 << 22, 2)))
+        pin_in = eng.host_array(dense[0][0].shape)
+        lat_p = []
+        for k in range(min(30, need)):
+            pin_in[:dense[k][0].shape[0]] = dense[k][0]
+            t0 = time.perf_counter()
+            m3.process_scan(pin_in[:dense[k][0].shape[0]], dense[k][1], out=pin_out)
+            lat_p.append(1e3 * (time.perf_counter() - t0))
+        plain_lat_pinned = {"p50": pct(lat_p[2:], 50), "p99": pct(lat_p[2:], 99), "scans": len(lat_p) - 2}
+        m3.close()
+        del pin_out, pin_in
+        log("full-call latency, pinned host arrays", plain_lat_pinned)
     m.close()
     buf.free()
     eng.close()
@@ -456,6 +470,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
         "latency_ms": {"plain_full_call": plain_lat,
+                       "plain_full_call_pinned": plain_lat_pinned,
                        "served": result.get("serving", {}).get("loguniform", {}).get("served"),
                        "plain_served": result.get("serving", {}).get("loguniform", {}).get("plain")},
     }

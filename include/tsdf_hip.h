@@ -280,6 +280,13 @@ int tsdf_copy_to_host(tsdf_engine* eng, void* dst, const void* src, int64_t byte
  * both pointers 16-byte aligned); synchronous.  Also serves as the
  * known-size reference for HBM byte counters. */
 int tsdf_device_copy(tsdf_engine* eng, void* dst, const void* src, int64_t bytes);
+/* Page-locked host memory (no reference counterpart: the reference's caller
+ * passes pageable arrays).  Host points and publish outputs placed here move
+ * over PCIe at DMA speed instead of through the runtime's staging copies,
+ * which for a large publish is most of the call.  The memory outlives the
+ * engine; tsdf_host_free takes no engine so that it can be released after. */
+int tsdf_host_alloc(tsdf_engine* eng, int64_t bytes, void** out);
+int tsdf_host_free(void* ptr);
 
 /* ---- input path ------------------------------------------------------------------
  * Replaces: Transformer::convertPointCloud2ToPointCloudXYZ

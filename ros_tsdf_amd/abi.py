@@ -124,6 +124,8 @@ SIGNATURES = {
                        C.POINTER(C.c_double), i32p, C.POINTER(ServeStats)],
     "tsdf_device_alloc": [vp, C.c_int64, C.POINTER(vp)],
     "tsdf_device_free": [vp, vp],
+    "tsdf_host_alloc": [vp, C.c_int64, C.POINTER(vp)],
+    "tsdf_host_free": [vp],
     "tsdf_copy_to_device": [vp, vp, vp, C.c_int64],
     "tsdf_copy_to_host": [vp, vp, vp, C.c_int64],
     "tsdf_device_copy": [vp, vp, vp, C.c_int64],

@@ -1259,6 +1259,25 @@ int tsdf_device_free(tsdf_engine* E, void* ptr) {
   return TSDF_OK;
 }
 
+int tsdf_host_alloc(tsdf_engine* E, int64_t bytes, void** out) {
+  if (!E || !out || bytes < 0) return fail(TSDF_ERR_INVALID_ARG, "bad argument");
+  *out = nullptr;
+  CHK(set_device(E));
+  void* p = nullptr;
+  const hipError_t e = hipHostMalloc(&p, (size_t)std::max<int64_t>(bytes, 1), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(TSDF_ERR_NO_MEMORY, "hipHostMalloc of %lld bytes failed: %s", (long long)bytes, hipGetErrorString(e));
+  }
+  *out = p;
+  return TSDF_OK;
+}
+
+int tsdf_host_free(void* ptr) {
+  if (ptr) HIPCHK(hipHostFree(ptr));
+  return TSDF_OK;
+}
+
 int tsdf_copy_to_device(tsdf_engine* E, void* dst, const void* src, int64_t bytes) {
   if (!E || (bytes > 0 && (!dst || !src)) || bytes < 0) return fail(TSDF_ERR_INVALID_ARG, "bad argument");
   CHK(set_device(E));

@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -167,6 +168,21 @@ class Engine:
 
     def buffer(self, nbytes) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
+
+    def host_array(self, shape, dtype=np.float32) -> np.ndarray:
+        """A zeroed numpy array in page-locked host memory (tsdf_host_alloc): publish outputs and host points
+        kept there cross PCIe without staging copies.  The memory is released with the last reference to the
+        array or its views, also after the engine is closed."""
+        dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * dtype.itemsize
+        p = C.c_void_p()
+        check(abi.load().tsdf_host_alloc(self.h, n, C.byref(p)))
+        raw = (C.c_ubyte * max(n, 1)).from_address(p.value)
+        fin = weakref.finalize(raw, abi.load().tsdf_host_free, p.value)
+        fin.atexit = False  # at process exit the memory goes with the process; no HIP call during shutdown
+        a = np.frombuffer(raw, dtype=np.uint8, count=n).view(dtype).reshape(shape)
+        a[...] = 0
+        return a
 
     def cloud2_to_points(self, data, width, height, point_step, row_step, offsets, out: DeviceBuffer,
                          is_bigendian=False, datatype=abi.TSDF_FIELD_FLOAT32, data_bytes=None) -> int:

@@ -366,6 +366,33 @@ def test_packed_state_publish_and_gc_sequence(oracle_lib, fmt):
         om.close()
 
 
+def test_pinned_host_arrays():
+    """Page-locked host arrays (Engine.host_array) as process_scan input and publish outputs give exactly
+    the voxels of pageable arrays; the arrays stay usable, and are released, after the engine is closed."""
+    import gc
+    from ros_tsdf_amd import Engine
+    pts, s = dense_scan(4, n=8192)
+    with Engine(heap_blocks=1 << 15, total_buckets=1 << 16, max_batch_points=1 << 14) as e:
+        a = e.map(D, buckets_log2=12)
+        b = e.map(D, buckets_log2=12)
+        pin_pts = e.host_array(pts.shape)
+        pin_pts[...] = pts
+        out = (e.host_array((1 << 18, 3)), e.host_array((1 << 18, 2)))
+        assert out[0].dtype == np.float32 and not out[0].any()
+        pos_a, data_a, n_a = a.process_scan(pin_pts, s, out=out)
+        pos_b, data_b, n_b = b.process_scan(pts, s)
+        assert n_a == n_b > 0 and np.shares_memory(pos_a, out[0])
+        ra = np.concatenate([pos_a, data_a], 1).view(np.uint32)
+        rb = np.concatenate([pos_b, data_b], 1).view(np.uint32)
+        assert np.array_equal(ra[np.lexsort(ra.T[::-1])], rb[np.lexsort(rb.T[::-1])])
+        a.close()
+        b.close()
+    keep = out[0][:n_a].copy()
+    assert np.array_equal(keep.view(np.uint32), pos_a.view(np.uint32))
+    del pin_pts, out, pos_a, data_a
+    gc.collect()
+
+
 def test_workgroup_block_table_overflow(oracle_lib):
     """More distinct blocks per 256-point workgroup than the binning kernel's LDS table holds (1024 slots):
     2048 points scattered over a 1 km cube, a 1.2 m truncation band (up to 12 blocks and 96 voxels per point).
