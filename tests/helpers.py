@@ -84,3 +84,11 @@ def compare_publish_to_golden(pos, sdf, w, expected, T, sdf_tol_T, w_tol_rel, po
     e_w = np.asarray([x for _, _, x in exp])[o]
     assert np.abs(np.asarray(sdf, np.float64)[g] - e_s).max() <= sdf_tol_T * T
     assert (np.abs(np.asarray(w, np.float64)[g] - e_w) <= w_tol_rel * e_w).all()
+
+
+# Publish / GC radius boundary case: 4 m blocks, publish and GC radius 8 m.  One point in each of blocks
+# (1,0,0), (2,0,0), (3,0,0), whose centres (6,2,2), (10,2,2), (14,2,2) lie exactly 4, 8 and 12 m from the
+# sensor in FP32 (squared distances 16, 64, 144).
+RADIUS_PARAMS = [0.5, 0.1, 1000.0, 64.0, 64.0]
+RADIUS_SENSOR = np.array([2.0, 2.0, 2.0], np.float32)
+RADIUS_POINTS = np.array([[6.25, 2.25, 2.25, 1], [10.25, 2.25, 2.25, 1], [14.25, 2.25, 2.25, 1]], np.float32)

@@ -103,6 +103,30 @@ def test_empty_scan_and_empty_batch(eng):
     buf.free()
 
 
+@pytest.mark.parametrize("fmt", [abi.TSDF_STATE_F32, abi.TSDF_STATE_F16, abi.TSDF_STATE_F16_QW])
+def test_publish_and_gc_radius_boundaries(oracle_lib, fmt):
+    """Blocks centred exactly 4, 8 and 12 m from the sensor with publish and GC radius 8 m: the block on
+    the radius is published (withinDistanceSquared is <=, H:750-757, H:812) and kept by GC (H:847); the
+    engine's sets equal the oracle's."""
+    from ros_tsdf_amd import Engine
+    from tests.helpers import RADIUS_PARAMS, RADIUS_POINTS, RADIUS_SENSOR
+    with Engine(heap_blocks=1 << 10, total_buckets=1 << 10, max_batch_points=1 << 10, state_format=fmt) as e:
+        m, om, _ = integrate_both(e, oracle_lib, RADIUS_PARAMS, [(RADIUS_POINTS, RADIUS_SENSOR)], buckets_log2=6)
+        k, _, _ = m.export()
+        assert sorted(map(tuple, k.tolist())) == [(1, 0, 0), (2, 0, 0), (3, 0, 0)]
+        pos, _, n = m.publish(RADIUS_SENSOR)
+        opos, _, _ = om.publish(RADIUS_SENSOR)
+        blocks = sorted({tuple(int(c) for c in np.floor(p / 4.0)) for p in pos.astype(np.float64)})
+        assert blocks == [(1, 0, 0), (2, 0, 0)]
+        rg, ro = pos.view(np.uint32), np.asarray(opos, np.float32).view(np.uint32)
+        assert np.array_equal(rg[np.lexsort(rg.T[::-1])], ro[np.lexsort(ro.T[::-1])])
+        assert m.gc(RADIUS_SENSOR) == om.gc(RADIUS_SENSOR) == 1
+        k, _, _ = m.export()
+        assert sorted(map(tuple, k.tolist())) == [(1, 0, 0), (2, 0, 0)]
+        m.close()
+        om.close()
+
+
 def test_misaligned_point_buffer_is_refused(eng):
     """Points are float4 rows: a device pointer off a 16-byte boundary is refused before anything runs."""
     import ctypes as C

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
+from tests.helpers import RADIUS_PARAMS, RADIUS_POINTS, RADIUS_SENSOR
 
 f32 = np.float32
 
@@ -286,6 +287,21 @@ def test_publish_and_gc_brute_force():
     assert removed == int((d2 > 900.0).sum())
     keys2, _, _ = om.export()
     assert {tuple(k) for k in keys2} == {tuple(k) for k in keys[d2 <= 900.0]}
+    om.close()
+
+
+def test_publish_and_gc_radius_boundaries():
+    """withinDistanceSquared is `<=` (H:750-757): a block centred exactly on the publish radius is published
+    (H:812), and one exactly on the GC radius is kept, since GC removes only blocks outside it (H:847)."""
+    om = oracle.OracleMap(RADIUS_PARAMS)
+    om.integrate(RADIUS_POINTS, RADIUS_SENSOR)
+    keys, _, _ = om.export()
+    assert sorted(map(tuple, keys.tolist())) == [(1, 0, 0), (2, 0, 0), (3, 0, 0)]
+    pos, _, _ = om.publish(RADIUS_SENSOR)
+    assert sorted({tuple(int(c) for c in np.floor(p / 4.0)) for p in pos.astype(np.float64)}) == [(1, 0, 0), (2, 0, 0)]
+    assert om.gc(RADIUS_SENSOR) == 1
+    keys, _, _ = om.export()
+    assert sorted(map(tuple, keys.tolist())) == [(1, 0, 0), (2, 0, 0)]
     om.close()
 
 
