@@ -414,7 +414,10 @@ __device__ __forceinline__ int peek_find(const HomePeek& h, unsigned long long k
   return e;
 }
 
-__global__ void __launch_bounds__(256) k_bin(Dev D) {
+// 168 VGPRs at most: three waves per SIMD where the grid has them (batched
+// passes; a 131k-point scan launches two per SIMD whatever the budget), with
+// a handful of spills off the walks.
+__global__ void __launch_bounds__(256, 3) k_bin(Dev D) {
   __shared__ unsigned long long tkey[kBinSlots];
   __shared__ int tent[kBinSlots];
   __shared__ uint32_t trec[kBinSlots];
@@ -553,17 +556,14 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
     }
     __syncthreads();
     const int n_listed = n_occ;
-    HomePeek pk0, pk1;
-    unsigned long long key0 = kEmptyKey, key1 = kEmptyKey;
+    // (one bucket per thread: a second one held through the walks would cost
+    // the registers of the third wave per SIMD)
+    HomePeek pk0;
+    unsigned long long key0 = kEmptyKey;
     if ((int)threadIdx.x < n_listed) {
       int ms;
       key0 = table_map_key(tkey[locc[threadIdx.x]], ms);
       pk0 = peek_home(D, D.maps[ms], key0);
-    }
-    if ((int)(threadIdx.x + blockDim.x) < n_listed) {
-      int ms;
-      key1 = table_map_key(tkey[locc[threadIdx.x + blockDim.x]], ms);
-      pk1 = peek_home(D, D.maps[ms], key1);
     }
     // ---- phase A2: voxel walks ----
     if (walk) {
@@ -663,10 +663,10 @@ __global__ void __launch_bounds__(256) k_bin(Dev D) {
     if (i < P) D.pair_n[i] = (uint8_t)nj;
     __syncthreads();
     // ---- phase B: one global lookup and one counter atomic per listed block
-    // (the first two per thread from the buckets read ahead) ----
+    // (the first per thread from the bucket read ahead) ----
     for (int y = threadIdx.x, r = 0; y < n_occ; y += blockDim.x, ++r) {
       int e = -1;
-      if (y < n_listed && r < 2) e = r == 0 ? peek_find(pk0, key0) : peek_find(pk1, key1);
+      if (y < n_listed && r == 0) e = peek_find(pk0, key0);
       resolve_slot(locc[y], e);
     }
     __syncthreads();
