@@ -66,12 +66,62 @@ def plain_throughput(eng, m, buf, pool, idx, with_stats=True):
     return total, pts, stats
 
 
+def full_size_batches(eng, bmaps_pts, bmaps_sens, knobs):
+    """Batched against plain per-scan on full-size scans (131,072 points each): 8 maps x 4 scans per launch,
+    three launches (the first untimed).  knobs: traverse_block, fold_grid, buckets_log2, max_chain."""
+    from ros_tsdf_amd import BatchGraph, configs
+
+    def mk():
+        return [eng.map(configs.DENSE_PARAMS, buckets_log2=knobs["buckets_log2"], max_chain=knobs["max_chain"])
+                for _ in range(configs.BATCH_MAPS)]
+
+    fscans = [(bmaps_pts[mi][k], bmaps_sens[mi][k], mi) for b in range(3) for mi in range(configs.BATCH_MAPS)
+              for k in range(4 * b, 4 * b + 4)]
+    fpool = make_pool(fscans)
+    fbuf = eng.buffer(fpool.points.nbytes)
+    fbuf.upload(fpool.points)
+    per = len(fscans) // 3
+    fdescs = [[(fpool.maps[j], fpool.counts[j], fpool.offsets[j], fpool.sensors[j]) for j in range(per * b, per * b + per)]
+              for b in range(3)]
+    eng.set_launch(traverse_block=knobs["traverse_block"], fold_grid=knobs["fold_grid"])
+    maps = mk()
+    g = BatchGraph(eng, maps, max_points=max(sum(x[1] for x in d) for d in fdescs), max_scans=per)
+    g.launch(fdescs[0], fbuf)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    for d in fdescs[1:]:
+        g.launch(d, fbuf)
+        eng.synchronize()
+    tfb = time.perf_counter() - t0
+    fpts = sum(sum(x[1] for x in d) for d in fdescs[1:])
+    g.close()
+    for mm in maps:
+        mm.close()
+    maps = mk()
+    eng.set_launch()
+    tfp = 0.0
+    for i, d in enumerate(fdescs):
+        for (mi, n, off, s) in d:
+            eng.synchronize()
+            t0 = time.perf_counter()
+            maps[mi].integrate(fbuf, n, s, offset_points=off)
+            eng.synchronize()
+            if i > 0:
+                tfp += time.perf_counter() - t0
+    for mm in maps:
+        mm.close()
+    fbuf.free()
+    return {"points_per_s": fpts / tfb, "plain_per_scan_points_per_s": fpts / tfp, "speedup_vs_plain": tfp / tfb,
+            "scans_per_launch": per, "points_per_scan": fpts / (2 * per), "ms_per_launch": 1e3 * tfb / 2}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed scans of the dense workload")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--only", default="all", choices=["all", "plain"])
+    ap.add_argument("--only", default="all", choices=["all", "plain", "batched_full"],
+                    help="plain: the headline only; batched_full: only the full-size batched comparison (profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--error-scans", type=int, default=200,
                     help="dense scans integrated before the FP16/FP32 error figures (also the CPU-baseline sample)")
@@ -84,6 +134,18 @@ def main():
         sys.exit(2)
 
     from ros_tsdf_amd import configs, workloads
+
+    if args.only == "batched_full":
+        from ros_tsdf_amd import Engine
+        bmaps_pts, bmaps_sens = workloads.generate_maps(configs.DENSE, configs.BATCH_MAPS, 12)
+        eng = Engine(heap_blocks=1 << 21, total_buckets=1 << 25, max_batch_points=1 << 22)
+        knobs = {"traverse_block": 256, "fold_grid": 8192, "buckets_log2": 18, "max_chain": 64}
+        r = full_size_batches(eng, bmaps_pts, bmaps_sens, knobs)
+        eng.close()
+        log("batched full-size", r)
+        print(json.dumps({"metric": "batched full-size points/s", "value": r["points_per_s"], "unit": "points/s",
+                          "n_gpus": 1, "higher_is_better": True, "knobs": knobs, "batched_full_size": r}))
+        return
 
     K, W = args.steps, args.warmup
     n_dense = configs.DENSE.n_scans
@@ -289,46 +351,8 @@ def main():
         for mm in maps:
             mm.close()
         # the same comparison with full-size scans (131,072 points each): 8 maps x 4 scans per launch
-        fscans = [(bmaps_pts[mi][k], bmaps_sens[mi][k], mi) for b in range(3) for mi in range(configs.BATCH_MAPS)
-                  for k in range(4 * b, 4 * b + 4)]
-        fpool = make_pool(fscans)
-        fbuf = eng.buffer(fpool.points.nbytes)
-        fbuf.upload(fpool.points)
-        per = len(fscans) // 3
-        fdescs = [[(fpool.maps[j], fpool.counts[j], fpool.offsets[j], fpool.sensors[j]) for j in range(per * b, per * b + per)]
-                  for b in range(3)]
-        eng.set_launch(traverse_block=best["traverse_block"], fold_grid=best["fold_grid"])
-        maps = mk()
-        g = BatchGraph(eng, maps, max_points=max(sum(x[1] for x in d) for d in fdescs), max_scans=per)
-        g.launch(fdescs[0], fbuf)
-        eng.synchronize()
-        t0 = time.perf_counter()
-        for d in fdescs[1:]:
-            g.launch(d, fbuf)
-            eng.synchronize()
-        tfb = time.perf_counter() - t0
-        fpts = sum(sum(x[1] for x in d) for d in fdescs[1:])
-        g.close()
-        for mm in maps:
-            mm.close()
-        maps = mk()
-        eng.set_launch()
-        tfp = 0.0
-        for i, d in enumerate(fdescs):
-            for (mi, n, off, s) in d:
-                eng.synchronize()
-                t0 = time.perf_counter()
-                maps[mi].integrate(fbuf, n, s, offset_points=off)
-                eng.synchronize()
-                if i > 0:
-                    tfp += time.perf_counter() - t0
-        result["batched_full_size"] = {"points_per_s": fpts / tfb, "plain_per_scan_points_per_s": fpts / tfp,
-                                       "speedup_vs_plain": tfp / tfb, "scans_per_launch": per,
-                                       "points_per_scan": fpts / (2 * per), "ms_per_launch": 1e3 * tfb / 2}
+        result["batched_full_size"] = full_size_batches(eng, bmaps_pts, bmaps_sens, best)
         log("batched full-size", result["batched_full_size"])
-        for mm in maps:
-            mm.close()
-        fbuf.free()
         bbuf.free()
         eng.close()
 
