@@ -240,6 +240,7 @@ __global__ void __launch_bounds__(256) k_pass_begin(Dev D, uint32_t mask, PassIn
       D.ctr->touched_n = 0;
       D.ctr->fail_n = 0;
       D.ctr->new_n = 0;
+      D.ctr->fold_next = 0;
     }
     if (t < FC_COUNT) D.ctr->fold_n[t] = 0;
     if (inl.nscans >= 0) {
@@ -1606,11 +1607,18 @@ __global__ void __launch_bounds__(256, 7) k_fold_a(Dev D) {
   stat_add(D, ST_UNIQUE, uniq);
 }
 
-// Fold, second launch: blocks of kCountTileS+1..kCountTile records, then the
-// larger ones; one 64 KB LDS buffer serves as the counting tile pair or the
-// sort tile.
+// Fold, second launch: blocks of kCountTileS+1..kCountTile records, and the
+// larger ones (folded in voxel ranges); one 64 KB LDS buffer serves as the
+// counting tile pair or the sort tile.
 // 1024 threads; eight waves per SIMD (two workgroups, and their 2 x 68 KB
 // of LDS, per CU) caps the kernel at 64 VGPRs.
+// Workgroup w first folds block w of the kCountTile class (the whole work
+// of a single scan's pass, whose descriptor load waits on no counter).  The
+// rest -- every larger block, then the kCountTile blocks from gridDim.x on --
+// is handed out one item at a time from a pass counter, largest first:
+// blocks differ in cost by an order of magnitude, and a fixed stride over
+// them left some workgroups with several of the largest while the rest had
+// finished.
 constexpr int kFoldBThreads = 1024;
 template <class State>
 __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
@@ -1620,17 +1628,29 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
   __shared__ uint32_t start[kVoxelsPerBlock + 1];
   __shared__ uint32_t sh[kScanSh];
   __shared__ uint32_t rstart[kMaxRanges + 1];
+  __shared__ int next_item;
   unsigned long long uniq = 0, dropped = 0;
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG2]);
   const int4* desc = D.fold_desc + D.fold_base[FC_WG2];
-  for (int q = blockIdx.x; q < n_wg; q += gridDim.x)
-    fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[q], buf, buf + kCountTile, cnt, start, sh, uniq);
+  if ((int)blockIdx.x < n_wg)
+    fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[blockIdx.x], buf, buf + kCountTile, cnt, start, sh, uniq);
   const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
+  const int n_rest = n_huge + max(n_wg - (int)gridDim.x, 0);
   const int4* hdesc = D.fold_desc + D.fold_base[FC_HUGE];
-  for (int q = blockIdx.x; q < n_huge; q += gridDim.x) {
-    const int4 d = hdesc[q];
-    if (!fold_ranged_block<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, rstart, sh, uniq))
-      fold_huge_block<State>(D, d, buf, uniq, dropped);
+  while (n_rest > 0) {
+    if (threadIdx.x == 0) next_item = atomicAdd(&D.ctr->fold_next, 1);
+    __syncthreads();
+    const int k = next_item;
+    __syncthreads();
+    if (k >= n_rest) break;
+    if (k < n_huge) {
+      const int4 d = hdesc[k];
+      if (!fold_ranged_block<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, rstart, sh, uniq))
+        fold_huge_block<State>(D, d, buf, uniq, dropped);
+    } else {
+      fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[(int)gridDim.x + k - n_huge], buf, buf + kCountTile,
+                                                          cnt, start, sh, uniq);
+    }
   }
   stat_add(D, ST_UNIQUE, uniq);
   stat_add(D, ST_UPD_DROPPED, dropped);
