@@ -1612,13 +1612,13 @@ __global__ void __launch_bounds__(256, 7) k_fold_a(Dev D) {
 // counting tile pair or the sort tile.
 // 1024 threads; eight waves per SIMD (two workgroups, and their 2 x 68 KB
 // of LDS, per CU) caps the kernel at 64 VGPRs.
-// Workgroup w first folds block w of the kCountTile class (the whole work
-// of a single scan's pass, whose descriptor load waits on no counter).  The
-// rest -- every larger block, then the kCountTile blocks from gridDim.x on --
-// is handed out one item at a time from a pass counter, largest first:
-// blocks differ in cost by an order of magnitude, and a fixed stride over
-// them left some workgroups with several of the largest while the rest had
-// finished.
+// Items are handed out one at a time from a pass counter, largest first
+// (the ranged blocks, then the kCountTile class): blocks differ in cost by an
+// order of magnitude, and a fixed stride over them left some workgroups with
+// several of the largest while the rest had finished.  A pass without ranged
+// blocks (a single scan's: fewer items than workgroups) folds block w of the
+// kCountTile class in workgroup w and touches the counter only for items
+// beyond the grid.
 constexpr int kFoldBThreads = 1024;
 template <class State>
 __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
@@ -1631,25 +1631,31 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
   __shared__ int next_item;
   unsigned long long uniq = 0, dropped = 0;
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG2]);
-  const int4* desc = D.fold_desc + D.fold_base[FC_WG2];
-  if ((int)blockIdx.x < n_wg)
-    fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[blockIdx.x], buf, buf + kCountTile, cnt, start, sh, uniq);
   const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
-  const int n_rest = n_huge + max(n_wg - (int)gridDim.x, 0);
+  const int4* desc = D.fold_desc + D.fold_base[FC_WG2];
   const int4* hdesc = D.fold_desc + D.fold_base[FC_HUGE];
-  while (n_rest > 0) {
-    if (threadIdx.x == 0) next_item = atomicAdd(&D.ctr->fold_next, 1);
+  int q0 = 0;  // the first item taken from the counter
+  if (n_huge == 0) {
+    if ((int)blockIdx.x < n_wg)
+      fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[blockIdx.x], buf, buf + kCountTile, cnt, start, sh, uniq);
+    q0 = (int)gridDim.x;
+  }
+  // items in order: the ranged blocks, then the kCountTile class; with ranged
+  // blocks present every item comes from the counter
+  const int total = n_huge + n_wg;
+  while (q0 < total) {
+    if (threadIdx.x == 0) next_item = q0 + atomicAdd(&D.ctr->fold_next, 1);
     __syncthreads();
     const int k = next_item;
     __syncthreads();
-    if (k >= n_rest) break;
+    if (k >= total) break;
     if (k < n_huge) {
       const int4 d = hdesc[k];
       if (!fold_ranged_block<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, rstart, sh, uniq))
         fold_huge_block<State>(D, d, buf, uniq, dropped);
     } else {
-      fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[(int)gridDim.x + k - n_huge], buf, buf + kCountTile,
-                                                          cnt, start, sh, uniq);
+      fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[k - n_huge], buf, buf + kCountTile, cnt, start, sh,
+                                                          uniq);
     }
   }
   stat_add(D, ST_UNIQUE, uniq);
