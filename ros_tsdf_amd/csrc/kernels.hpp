@@ -1365,14 +1365,16 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
 
 // One block beyond the counting tile, folded in voxel ranges: the voxel
 // counts of the whole block are taken first (a histogram over its records,
-// read from the segment), and a voxel belongs to range start / kRangeSpan,
-// where start is the number of the block's records in the voxels before it.
-// While no voxel holds more than kRangeSpan records, the ranges follow each
-// other without gaps and each holds fewer than 2 kRangeSpan <= kCountTile
+// read from the segment), and a voxel belongs to range start / span, where
+// start is the number of the block's records in the voxels before it and
+// span = kCountTile - (the block's largest voxel count).  While no voxel
+// holds more than kRangeSpan records, span >= kRangeSpan, the ranges follow
+// each other without gaps and each holds fewer than span + max = kCountTile
 // records, so each range is sorted and folded like a counting tile, its
-// records taken from the segment once per range; voxel l's state is folded
-// by thread l in its range.  Returns false (with nothing folded) for a block
-// outside those bounds, which then takes the merge path.
+// records taken from the segment once per range (the widest span that fits
+// keeps those passes few); voxel l's state is folded by thread l in its
+// range.  Returns false (with nothing folded) for a block outside those
+// bounds, which then takes the merge path.
 constexpr int kRangeSpan = kCountTile / 2;
 constexpr int kMaxRanges = 512;
 template <class State, int NT>
@@ -1381,24 +1383,29 @@ __device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, un
                                                   uint32_t* rstart, uint32_t* sh, unsigned long long& uniq) {
   static_assert(NT >= kVoxelsPerBlock, "a thread per voxel");
   static_assert(2 * kRangeSpan <= kCountTile, "a range fits the counting tile");
-  __shared__ int too_big, nranges_sh;
+  __shared__ int too_big, nranges_sh, maxc_sh;
   const int c = d.y;
   const unsigned long long* sg = D.seg + (uint32_t)d.x;
   const int t = threadIdx.x;
   if (t < kVoxelsPerBlock) cnt[t] = 0;
-  if (t == 0) too_big = c / kRangeSpan + 1 > kMaxRanges ? 1 : 0;
+  if (t == 0) {
+    too_big = c / kRangeSpan + 1 > kMaxRanges ? 1 : 0;
+    maxc_sh = 0;
+  }
   __syncthreads();
   for (int x = t; x < c; x += NT) atomicAdd(&cnt[rec_voxel(sg[x])], 1u);
   __syncthreads();
   const uint32_t mine = t < kVoxelsPerBlock ? cnt[t] : 0u;
   if (mine > (uint32_t)kRangeSpan) too_big = 1;
+  if (mine != 0) atomicMax(&maxc_sh, (int)mine);
   uint32_t tot;
-  const uint32_t ex = block_scan_excl(mine, sh, tot);  // (its barriers publish too_big)
+  const uint32_t ex = block_scan_excl(mine, sh, tot);  // (its barriers publish too_big and maxc_sh)
   if (too_big) return false;
+  const uint32_t span = (uint32_t)(kCountTile - maxc_sh);  // >= kRangeSpan
   // voxel t: its start, range, state.  The ranges are those of the voxels
   // with records: 0 up to the range of the last such voxel, with no gap
-  // (consecutive voxels' starts differ by at most kRangeSpan).
-  const int rg = (int)(ex / kRangeSpan);
+  // (consecutive voxels' starts differ by at most the largest count <= span).
+  const int rg = (int)(ex / span);
   if (t < kVoxelsPerBlock) start[t] = ex;
   if (t == 0) start[kVoxelsPerBlock] = tot;
   if (mine != 0 && ex + mine == tot) nranges_sh = rg + 1;
@@ -1407,11 +1414,12 @@ __device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, un
   // range r starts at the start of its first voxel: the voxel whose range
   // differs from the previous voxel's
   const int nranges = nranges_sh;
-  if (t < kVoxelsPerBlock && (t == 0 || (int)(start[t - 1] / kRangeSpan) != rg) && ex < tot) rstart[rg] = ex;
+  if (t < kVoxelsPerBlock && (t == 0 || (int)(start[t - 1] / span) != rg) && ex < tot) rstart[rg] = ex;
   if (t == 0) rstart[nranges] = tot;
   __syncthreads();
   for (int r = 0; r < nranges; ++r) {
     const uint32_t base = rstart[r];
+    const uint32_t lo = (uint32_t)r * span;
     const int nr = (int)(rstart[r + 1] - base);
     if (t < kVoxelsPerBlock) cnt[t] = 0;
     __syncthreads();
@@ -1419,7 +1427,7 @@ __device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, un
       const unsigned long long rec = sg[x];
       const int l = rec_voxel(rec);
       const uint32_t sl = start[l];
-      if ((int)(sl / kRangeSpan) == r) rout[sl - base + atomicAdd(&cnt[l], 1u)] = rec;
+      if (sl - lo < span) rout[sl - base + atomicAdd(&cnt[l], 1u)] = rec;  // (unsigned: sl >= lo too)
     }
     __syncthreads();
     // point order inside each voxel (as in fold_count_block)
