@@ -69,28 +69,34 @@ def test_batch_equals_per_scan(eng):
 
 def test_batched_dense_pass_equals_per_scan():
     """Four full-size dense scans of each of two maps in one pass: thousands of blocks of more than 1024
-    records (the per-scan count times four), more than the large-block fold has workgroups, so most are
-    handed out by its pass counter.  Each map equals the per-scan path's, bit for bit, and a second batched
-    pass (the counter starts again from zero) keeps them equal."""
-    from ros_tsdf_amd import Engine
-    bp, bs = workloads.generate_maps(configs.DENSE, 2, 8, workers=1)  # no fork once the GPU is in use
+    records (the per-scan count times four), more than the large-block fold has workgroups, so they are
+    handed out by its pass counter.  One eager batched pass, then two replays of the captured pass (the
+    counter is cleared inside the graph): after each, every map equals the per-scan path's, bit for bit."""
+    from ros_tsdf_amd import BatchGraph, Engine
+    bp, bs = workloads.generate_maps(configs.DENSE, 2, 12, workers=1)  # no fork once the GPU is in use
     with Engine(heap_blocks=1 << 18, total_buckets=1 << 20, max_batch_points=1 << 20) as e:
         mb = [e.map(configs.DENSE_PARAMS, buckets_log2=16) for _ in range(2)]
         ms = [e.map(configs.DENSE_PARAMS, buckets_log2=16) for _ in range(2)]
-        for half in range(2):
-            scans = [(bp[mi][4 * half + k], bs[mi][4 * half + k], mi) for mi in range(2) for k in range(4)]
+        g = BatchGraph(e, mb, max_points=8 * 131072, max_scans=8)
+        for third in range(3):
+            scans = [(bp[mi][4 * third + k], bs[mi][4 * third + k], mi) for mi in range(2) for k in range(4)]
             pool = workloads.Pool(scans)
             buf = e.buffer(pool.points.nbytes)
             buf.upload(pool.points)
             descs = [(pool.maps[i], pool.counts[i], pool.offsets[i], pool.sensors[i]) for i in range(len(pool))]
-            e.integrate_batch(mb, descs, buf)
+            if third == 0:
+                e.integrate_batch(mb, descs, buf)
+            else:
+                g.launch(descs, buf)
+            e.synchronize()
             assert e.stats()["voxel_updates"] > 8 * 1000000
             for (mi, n, off, s) in descs:
                 ms[mi].integrate(buf, n, s, offset_points=off)
             e.synchronize()
             buf.free()
             for a, b in zip(mb, ms):
-                assert exports_equal(a, b)
+                assert exports_equal(a, b), "pass %d" % third
+        g.close()
         for m in mb + ms:
             m.close()
 
