@@ -1,8 +1,6 @@
 """Shared comparison helpers: engine state vs the FP64 oracle."""
 import numpy as np
 
-from ros_tsdf_amd import configs
-
 
 def compare_maps(gmap, omap, T, sdf_tol_T, w_tol):
     """Block keys and touched voxels bit-exact; sdf within sdf_tol_T * T; weight within w_tol.

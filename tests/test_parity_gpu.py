@@ -4,7 +4,6 @@ Bar: block keys, touched-voxel sets and published positions bit-exact; SDF
 within SDF_TOL_T * truncation and weight within WEIGHT_TOL_REL * max_weight
 (ros_tsdf_amd.configs).
 """
-import numpy as np
 import pytest
 
 from ros_tsdf_amd import configs, synth
