@@ -35,10 +35,10 @@
 namespace tsdfhip {
 
 constexpr int kFoldThreads = 256;
-constexpr int kCountTileS = 1024; // records per LDS tile of the small k_fold_count (2 x 8 KB)
-constexpr int kCountTile = 4096;  // records per LDS tile of the large k_fold_count (2 x 32 KB)
-constexpr int kFoldCap = 8192;    // records per LDS sort tile of k_fold_huge (64 KB)
-constexpr int kMaxChunks = 32;    // sort tiles per block in k_fold_huge
+constexpr int kCountTileS = 1024; // records per LDS counting tile of k_fold_a's workgroup tier (2 x 8 KB)
+constexpr int kCountTile = 4096;  // records per LDS counting tile of k_fold_b (2 x 32 KB)
+constexpr int kFoldCap = 8192;    // records per LDS sort tile of the merge path (fold_huge_block, 64 KB)
+constexpr int kMaxChunks = 32;    // sort tiles per block on the merge path
 constexpr int kScanTile = 1024;  // elements per workgroup in the scan kernels
 
 // Size classes of touched blocks (records per block): the fold kernel and
@@ -844,7 +844,7 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int32_t* n_ptr, Count 
 }
 
 // First kernel after binning: per-tile record totals of the touched blocks
-// (as k_scan_tiles with CountTouchedRecords), and heap blocks for the pass's
+// (the tile sums of an exclusive scan over the touched list), and heap blocks for the pass's
 // new entries, 256 per workgroup and chunk (the chunks are taken from the
 // last workgroup down, so the tile workgroups get them last).  Heap blocks
 // come off the free stack with one pop, and one per-map count, per wave.
@@ -1009,7 +1009,7 @@ __device__ __forceinline__ int rec_voxel(unsigned long long r) { return (int)(r 
 
 
 // Record segments and fold descriptors of the touched blocks.  Per tile of
-// the touched list (tile totals from k_scan_tiles): the tile's carry is the
+// the touched list (tile totals from k_pass_scan): the tile's carry is the
 // sum of the preceding totals (summed by the workgroup itself, so no
 // separate top-level scan is launched), each block's segment offset follows
 // by a workgroup scan, and its 16-byte descriptor {record base, records,
