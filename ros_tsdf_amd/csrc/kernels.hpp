@@ -906,19 +906,22 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
   const int64_t stride = D.pcap;
   // run rows that exist in the pair arrays (they hold hdr.stride rows)
   const int rows = min(kEmitRuns, hdr.stride);
+  const bool wide = P > (1 << 18);  // a pass of more than two 131k-point scans
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
     // absolute record offsets of the point's runs, loaded before the walk
     // (the first kEmitRuns in registers; later runs, which need a walk far
     // longer than the truncation band of the workloads, are looked up when
     // reached).  The run rows are requested together with the run count, so
-    // the blocks' record bases are one dependent load away.
+    // the blocks' record bases are one dependent load away -- in a large
+    // (batched) pass only the first three rows: a point has 2-3 runs, and
+    // the unused rows would be most of the kernel's reads.
     const int nj = D.pair_n[i];
     uint32_t pe[kEmitRuns], pr[kEmitRuns];
 #pragma unroll
     for (int r = 0; r < kEmitRuns; ++r) {
       pe[r] = kPairFail;
       pr[r] = 0;
-      if (r < rows) {
+      if (r < rows && (!wide || r < 3 || r < nj)) {
         pe[r] = D.pair_e[(int64_t)r * stride + i];
         pr[r] = D.pair_r[(int64_t)r * stride + i];
       }
