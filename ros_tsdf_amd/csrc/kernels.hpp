@@ -37,8 +37,7 @@ namespace tsdfhip {
 constexpr int kFoldThreads = 256;
 constexpr int kCountTileS = 1024; // records per LDS counting tile of k_fold_a's workgroup tier (2 x 8 KB)
 constexpr int kCountTile = 4096;  // records per LDS counting tile of k_fold_b (2 x 32 KB)
-constexpr int kFoldCap = 8192;    // records per LDS sort tile of the merge path (fold_huge_block, 64 KB)
-constexpr int kMaxChunks = 32;    // sort tiles per block on the merge path
+constexpr int kFoldCap = 8192;    // records of k_fold_b's LDS buffer (64 KB: the counting tile pair)
 constexpr int kScanTile = 1024;  // elements per workgroup in the scan kernels
 
 // Size classes of touched blocks (records per block): the fold kernel and
@@ -1009,6 +1008,7 @@ __device__ __forceinline__ void weight_of(float d, float& w, float& wd) {
 }
 
 __device__ __forceinline__ int rec_voxel(unsigned long long r) { return (int)(r >> 55); }
+__device__ __forceinline__ int rec_point(unsigned long long r) { return (int)((r >> 32) & ((1u << kPointIdxBits) - 1u)); }
 
 
 // Record segments and fold descriptors of the touched blocks.  Per tile of
@@ -1377,7 +1377,7 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
 // records taken from the segment once per range (the widest span that fits
 // keeps those passes few); voxel l's state is folded by thread l in its
 // range.  Returns false (with nothing folded) for a block outside those
-// bounds, which then takes the merge path.
+// bounds, which then takes the point rounds of fold_point_rounds.
 constexpr int kRangeSpan = kCountTile / 2;
 constexpr int kMaxRanges = 512;
 template <class State, int NT>
@@ -1471,89 +1471,139 @@ __device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, un
   return true;
 }
 
-// One block beyond the counting tile: records sorted in LDS tiles of
-// kFoldCap written back in place, then every thread merges its voxel's runs
-// across the tiles in point order.  sk holds kFoldCap records.
-template <class State>
-__device__ __forceinline__ void fold_huge_block(const Dev& D, const int4 d, unsigned long long* sk,
-                                                unsigned long long& uniq, unsigned long long& dropped) {
+// One block that no voxel-range split fits: some voxel holds more than
+// kRangeSpan records (any block size).  The block is folded in rounds of
+// consecutive point indices [lo, hi), each holding at most kCountTile
+// records; one point has at most 22 records in a block (the longest
+// monotone cell path through 8^3 voxels), so every round advances.  A round
+// chooses hi with two histogram levels over the point index (bins of 1024
+// points, then the single points of the bin where the tile fills up),
+// gathers its records into the tile and sorts them as integers, which
+// orders them by voxel, then by point.  Thread l keeps voxel l's state in
+// registers from the first round to the last, so each voxel's updates are
+// folded in point order across the rounds.  The block's records are read
+// three times per round: this path is for inputs that pile thousands of
+// updates of one pass onto single voxels, which scans of a moving lidar do
+// not produce.  rin holds kCountTile records; rout's memory holds the
+// histograms.
+constexpr int kRoundBinShift = 10;
+constexpr int kRoundBins = 1 << (kPointIdxBits - kRoundBinShift);  // first-level bins over the pass's points
+template <class State, int NT>
+__device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long* rin, unsigned long long* rout,
+                                  uint32_t* cnt, uint32_t* start, uint32_t* sh, unsigned long long& uniq) {
+  static_assert(NT == (1 << kRoundBinShift), "a thread per point of a second-level bin");
+  static_assert(NT >= kVoxelsPerBlock, "a thread per voxel");
+  static_assert(kRoundBins * sizeof(uint32_t) <= kCountTile * sizeof(unsigned long long), "histogram fits in rout");
+  static_assert(kRoundBins % NT == 0, "first-level bins per thread");
+  __shared__ int lo_sh, hi_sh, bin_sh, base_sh, ntile_sh;
   const int c = d.y;
-  const int slot = d.z;
-  const int nch = (c + kFoldCap - 1) / kFoldCap;
+  const unsigned long long* sg = D.seg + (uint32_t)d.x;
+  const int t = threadIdx.x;
+  uint32_t* hist = reinterpret_cast<uint32_t*>(rout);
+  // the voxels with records, and their states
+  if (t < kVoxelsPerBlock) cnt[t] = 0;
+  if (t == 0) lo_sh = 0;
   __syncthreads();
-  if (nch > kMaxChunks) {
-    if (threadIdx.x == 0) dropped += c;
-    return;
-  }
-  unsigned long long* sg = D.seg + (uint32_t)d.x;
+  for (int x = t; x < c; x += NT) atomicAdd(&cnt[rec_voxel(sg[x])], 1u);
+  __syncthreads();
   const MapDev& m = D.maps[d.w];
-  const float maxW = m.maxW;
   const Codec cd = make_codec(m);
-  for (int ch = 0; ch < nch; ++ch) {
-    const int len = min(kFoldCap, c - ch * kFoldCap);
-    const int n2 = next_pow2(len);
-    for (int x = threadIdx.x; x < n2; x += blockDim.x) sk[x] = x < len ? sg[ch * kFoldCap + x] : ~0ull;
+  const bool h = t < kVoxelsPerBlock && cnt[t] != 0;
+  float S = 0.f, W = 0.f;
+  if (h) State::load(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, S, W);
+  constexpr int kPer = kRoundBins / NT;
+  for (;;) {
+    // level 1: records of points >= lo, per bin of 1024 points
+    for (int x = t; x < kRoundBins; x += NT) hist[x] = 0;
     __syncthreads();
-    bitonic_sort_lds(sk, n2);
-    for (int x = threadIdx.x; x < len; x += blockDim.x) sg[ch * kFoldCap + x] = sk[x];
-    __threadfence();
+    const int lo = lo_sh;
+    for (int x = t; x < c; x += NT) {
+      const int p = rec_point(sg[x]);
+      if (p >= lo) atomicAdd(&hist[p >> kRoundBinShift], 1u);
+    }
+    __syncthreads();
+    uint32_t v[kPer], sum = 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      v[u] = hist[t * kPer + u];
+      sum += v[u];
+    }
+    uint32_t tot;
+    uint32_t run = block_scan_excl(sum, sh, tot);
+    if (tot == 0) break;  // (tot is the same for every thread)
+    if (t == 0) {
+      bin_sh = -1;
+      hi_sh = 1 << kPointIdxBits;  // past every point
+    }
+    __syncthreads();
+    if (tot > (uint32_t)kCountTile) {
+      // the bin in which the tile fills up: exactly one
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        if (run <= (uint32_t)kCountTile && run + v[u] > (uint32_t)kCountTile) {
+          bin_sh = t * kPer + u;
+          base_sh = (int)run;
+        }
+        run += v[u];
+      }
+    }
+    __syncthreads();
+    const int B = bin_sh;
+    if (B >= 0) {
+      // level 2: the records of each point of bin B; hi is the first point
+      // whose records no longer fit
+      const int b0 = B << kRoundBinShift;
+      hist[t] = 0;
+      __syncthreads();
+      for (int x = t; x < c; x += NT) {
+        const int p = rec_point(sg[x]);
+        if (p >= lo && (p >> kRoundBinShift) == B) atomicAdd(&hist[p - b0], 1u);
+      }
+      __syncthreads();
+      const uint32_t w = hist[t];
+      uint32_t tot2;
+      const uint32_t e = block_scan_excl(w, sh, tot2) + (uint32_t)base_sh;
+      if (e <= (uint32_t)kCountTile && e + w > (uint32_t)kCountTile) hi_sh = b0 + t;
+      __syncthreads();
+    }
+    const int hi = hi_sh;
+    // the round's records, then each voxel's run in the sorted tile
+    if (t < kVoxelsPerBlock) cnt[t] = 0;
+    if (t == 0) ntile_sh = 0;
+    __syncthreads();
+    for (int x = t; x < c; x += NT) {
+      const unsigned long long r = sg[x];
+      const int p = rec_point(r);
+      if (p >= lo && p < hi) {
+        rin[atomicAdd(&ntile_sh, 1)] = r;
+        atomicAdd(&cnt[rec_voxel(r)], 1u);
+      }
+    }
+    __syncthreads();
+    const int nr = ntile_sh;
+    const int n2 = next_pow2(nr);
+    for (int x = nr + t; x < n2; x += NT) rin[x] = ~0ull;
+    {
+      const uint32_t cv = t < kVoxelsPerBlock ? cnt[t] : 0u;
+      uint32_t tot3;
+      const uint32_t ex = block_scan_excl(cv, sh, tot3);  // (its barriers also order the padding)
+      if (t < kVoxelsPerBlock) start[t] = ex;
+      if (t == 0) start[kVoxelsPerBlock] = tot3;
+    }
+    bitonic_sort_lds(rin, n2);
+    if (h) {
+      const int e = (int)start[t + 1];
+      for (int i = (int)start[t]; i < e; ++i) {
+        float w, wd;
+        weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
+        fold_step(S, W, w, wd, m.maxW);
+      }
+    }
+    if (t == 0) lo_sh = hi;
     __syncthreads();
   }
-  // Each tile's run of voxel l is kept as (begin | end << 16), offsets
-  // inside the tile, in registers indexed by compile-time constants only
-  // (an array indexed at run time would live in scratch memory).
-  static_assert(kFoldCap < 65536, "tile offsets in 16 bits");
-  for (int l = threadIdx.x; l < kVoxelsPerBlock; l += blockDim.x) {
-    uint32_t run[kMaxChunks];
-    bool any = false;
-#pragma unroll
-    for (int ch = 0; ch < kMaxChunks; ++ch) {
-      run[ch] = 0;
-      if (ch < nch) {
-        const int lo0 = ch * kFoldCap, hi0 = min(c, lo0 + kFoldCap);
-        int lo = lo0, hi = hi0;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (rec_voxel(ld_acq(&sg[mid])) < l) lo = mid + 1;
-          else hi = mid;
-        }
-        int lo2 = lo;
-        hi = hi0;
-        while (lo2 < hi) {
-          const int mid = (lo2 + hi) >> 1;
-          if (rec_voxel(ld_acq(&sg[mid])) < l + 1) lo2 = mid + 1;
-          else hi = mid;
-        }
-        run[ch] = (uint32_t)(lo - lo0) | ((uint32_t)(lo2 - lo0) << 16);
-        any = any || (lo2 > lo);
-      }
-    }
-    if (!any) continue;
-    const int64_t vi = (int64_t)slot * kVoxelsPerBlock + l;
-    float S, W;
-    State::load(D.heap, vi, cd, S, W);
-    for (;;) {
-      int best = -1;
-      unsigned long long bv = ~0ull;
-#pragma unroll
-      for (int ch = 0; ch < kMaxChunks; ++ch) {
-        const uint32_t b = run[ch] & 0xFFFFu, e = run[ch] >> 16;
-        if (b < e) {
-          const unsigned long long v = ld_acq(&sg[ch * kFoldCap + (int)b]);
-          if (best < 0 || v < bv) {
-            best = ch;
-            bv = v;
-          }
-        }
-      }
-      if (best < 0) break;
-      float w, wd;
-      weight_of(__uint_as_float((uint32_t)bv), w, wd);
-      fold_step(S, W, w, wd, maxW);
-#pragma unroll
-      for (int ch = 0; ch < kMaxChunks; ++ch) run[ch] += ch == best ? 1u : 0u;
-    }
-    State::store(D.heap, vi, cd, S, W);
+  if (h) {
+    State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, S, W);
     uniq++;
   }
   __syncthreads();
@@ -1619,8 +1669,8 @@ __global__ void __launch_bounds__(256, 7) k_fold_a(Dev D) {
 }
 
 // Fold, second launch: blocks of kCountTileS+1..kCountTile records, and the
-// larger ones (folded in voxel ranges); one 64 KB LDS buffer serves as the
-// counting tile pair or the sort tile.
+// larger ones (folded in voxel ranges, or in point rounds when a voxel holds
+// more than a range); one 64 KB LDS buffer serves as the counting tile pair.
 // 1024 threads; eight waves per SIMD (two workgroups, and their 2 x 68 KB
 // of LDS, per CU) caps the kernel at 64 VGPRs.
 // Items are handed out one at a time from a pass counter, largest first
@@ -1640,7 +1690,7 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
   __shared__ uint32_t sh[kScanSh];
   __shared__ uint32_t rstart[kMaxRanges + 1];
   __shared__ int next_item;
-  unsigned long long uniq = 0, dropped = 0;
+  unsigned long long uniq = 0;
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG2]);
   const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
   const int4* desc = D.fold_desc + D.fold_base[FC_WG2];
@@ -1663,14 +1713,13 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
     if (k < n_huge) {
       const int4 d = hdesc[k];
       if (!fold_ranged_block<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, rstart, sh, uniq))
-        fold_huge_block<State>(D, d, buf, uniq, dropped);
+        fold_point_rounds<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, sh, uniq);
     } else {
       fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[k - n_huge], buf, buf + kCountTile, cnt, start, sh,
                                                           uniq);
     }
   }
   stat_add(D, ST_UNIQUE, uniq);
-  stat_add(D, ST_UPD_DROPPED, dropped);
 }
 
 // ---- publish ---------------------------------------------------------------------------
