@@ -33,6 +33,14 @@ extern "C" {
 #define TSDF_ERR_CAPACITY (-4)     /* pool / table / batch capacity exceeded */
 #define TSDF_ERR_BUDGET (-5)       /* map admission refused by the heap budget */
 #define TSDF_ERR_DEADLINE (-6)     /* serving: request finished after its deadline */
+/* The work completed, but the last integration pass could not store every
+ * update: blocks were refused because a hash chain or the overflow-bucket
+ * pool was full, or the heap or the map's block quota was exhausted (the
+ * reference only prints on heap overflow, tsdf_handler.cu:43-45).  Every
+ * block that was stored holds all of its updates; the refused blocks' updates
+ * are lost.  tsdf_last_stats fills in the counts (insert_failures,
+ * heap_failures, quota_failures, updates_dropped) and returns this code. */
+#define TSDF_ERR_DROPPED (-7)
 
 /* Voxel state formats of an engine's block heap. */
 #define TSDF_STATE_F32 0  /* {float sdf, float weight}: the reference Voxel, CH:40-46 */
@@ -176,11 +184,16 @@ int tsdf_gc(tsdf_map* map, const float sensor[3], int64_t* removed);
 /* Replaces: TSDFHandler::processPointCloudAndUpdateVoxels (H:974-1016): the
  * whole per-scan sequence integrate -> publish -> gc with host inputs and
  * outputs.  points_host: n float4 rows; pos/data as for tsdf_publish.
- * Synchronous. */
+ * Synchronous.  Returns TSDF_ERR_DROPPED when the scan's integration dropped
+ * updates; publish and gc still ran and the outputs are complete. */
 int tsdf_process_scan(tsdf_map* map, const float* points_host, int64_t n, const float sensor[3],
                       float* pos, float* data, int64_t capacity, int64_t* count);
 
-/* Counters of the last integrate/batch/publish/gc call on this engine (synchronises). */
+/* Counters of the last integrate/batch/publish/gc call on this engine
+ * (synchronises).  Returns TSDF_ERR_DROPPED, with *out filled in, when the
+ * last integration pass dropped updates; integration calls themselves are
+ * asynchronous, so this (or tsdf_process_scan's status) is where a caller
+ * learns of it. */
 int tsdf_last_stats(tsdf_engine* eng, tsdf_scan_stats* out);
 
 /* ---- batched path ------------------------------------------------------------
@@ -193,8 +206,11 @@ int tsdf_integrate_batch(tsdf_engine* eng, tsdf_map* const* maps, int32_t nmaps,
                          const tsdf_scan_desc* scans, int32_t nscans, const float* points_dev);
 
 /* A batch pipeline captured once into a HIP graph for up to max_points
- * points and max_scans scans over a fixed set of maps, then replayed with new
- * scan descriptors and point buffers. */
+ * points and max_scans scans over a fixed set of maps (all of eng), then
+ * replayed with new scan descriptors and point buffers.  The graph does not
+ * keep its maps alive: a map may be destroyed while the graph exists, and a
+ * later launch with a scan of that map returns TSDF_ERR_INVALID_ARG without
+ * enqueueing anything (launches with scans of the remaining maps still run). */
 typedef struct tsdf_batch_graph tsdf_batch_graph;
 int tsdf_batch_graph_create(tsdf_engine* eng, tsdf_map* const* maps, int32_t nmaps,
                             int64_t max_points, int32_t max_scans, tsdf_batch_graph** out);
@@ -232,7 +248,8 @@ int tsdf_timing_read(tsdf_engine* eng, double* ms, int64_t* launches, int reset)
  * (scans of resident maps) become eligible at their arrival time; while the
  * device works on earlier batches, newly arrived requests are gathered into
  * the next batch (at most max_batch_scans scans and max_batch_points points,
- * one scan per map per batch so that scans of a map keep their order).  Up to
+ * oldest first; a batch folds the scans of one map in arrival order, so each
+ * map's scans keep their order).  Up to
  * max_inflight batches are queued on the device.  Arrival times are seconds
  * relative to the start of the run; the loop sleeps until arrivals.
  * latency_ms[i] receives completion minus arrival of request i, and

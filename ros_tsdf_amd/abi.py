@@ -17,6 +17,7 @@ TSDF_ERR_NO_MEMORY = -3
 TSDF_ERR_CAPACITY = -4
 TSDF_ERR_BUDGET = -5
 TSDF_ERR_DEADLINE = -6
+TSDF_ERR_DROPPED = -7
 
 TSDF_STATE_F32 = 0
 TSDF_STATE_F16 = 1
@@ -25,6 +26,10 @@ TSDF_STATE_F16_QW = 2
 TSDF_FIELD_FLOAT32 = 7
 
 STAGES = ["traverse", "assign", "scan", "scatter", "fold", "publish", "gc", "total"]
+
+
+class DroppedUpdatesWarning(RuntimeWarning):
+    """A scan ran to the end but its map could not store every update (TSDF_ERR_DROPPED)."""
 
 
 class TsdfError(RuntimeError):

@@ -52,6 +52,8 @@ constexpr int kMaxStride = 128;    // largest MAX_VOXELS_TRAVERSED a map may hav
 constexpr int kTimingRing = 256;
 // statistics an integration pass produces (publish and gc keep their own)
 constexpr uint32_t kPassStats = ((1u << ST_COUNT) - 1u) & ~((1u << ST_PUBLISHED) | (1u << ST_GC_REMOVED));
+// the pass statistics that mean updates were lost
+constexpr uint32_t kDropStats = (1u << ST_INSERT_FAIL) | (1u << ST_HEAP_FAIL) | (1u << ST_UPD_DROPPED);
 
 struct DescSlot {
   PassHdr hdr;
@@ -95,6 +97,7 @@ struct tsdf_batch_graph;
 struct tsdf_map {
   tsdf_engine* eng = nullptr;
   int slot = -1;
+  uint64_t id = 0;  // unique per engine: a slot reused by a later map gets a new id
   MapDev desc{};
   tsdf_params params{};
   int64_t quota_promised = 0;
@@ -158,6 +161,7 @@ struct tsdf_engine {
 
   // maps
   tsdf_map* map_by_slot[kMaxMaps] = {};
+  uint64_t next_map_id = 1;
   std::vector<std::pair<int64_t, int64_t>> free_ranges;  // bucket ranges [start, len)
   int64_t quota_promised = 0;
 
@@ -219,7 +223,11 @@ struct tsdf_engine {
 
 struct tsdf_batch_graph {
   tsdf_engine* eng = nullptr;
-  std::vector<tsdf_map*> maps;
+  // the graph's maps by slot and id: a launch checks that each map it uses
+  // still exists (a destroyed map's slot is either empty or holds a map with
+  // another id), and never touches a tsdf_map that may have been freed
+  std::vector<int> slots;
+  std::vector<uint64_t> ids;
   int64_t max_points = 0;
   int max_scans = 0;
   int stride = 0;
@@ -671,6 +679,7 @@ int map_create_impl(tsdf_engine* E, const tsdf_params* params, const tsdf_map_co
     return r;
   }
   E->map_by_slot[slot] = m;
+  m->id = E->next_map_id++;
   m->quota_promised = quota_promise;
   E->quota_promised += quota_promise;
   *out = m;
@@ -1031,10 +1040,24 @@ int tsdf_process_scan(tsdf_map* m, const float* points_host, int64_t n, const fl
   hipError_t e = hipMemcpyAsync(dpts, points_host, sizeof(float) * 4 * n, hipMemcpyHostToDevice, E->stream);
   if (e != hipSuccess) r = fail(TSDF_ERR_HIP, "H2D copy failed: %s", hipGetErrorString(e));
   if (!r) r = tsdf_integrate(m, dpts, n, sensor);
+  if (!r) {
+    // the pass's failure counters, reduced while the stream is still in order
+    // (publish and gc keep their own statistics)
+    hipLaunchKernelGGL(k_stats_reduce, dim3(ST_COUNT), dim3(256), 0, E->stream, E->view(), kDropStats);
+    e = hipGetLastError();
+    if (e != hipSuccess) r = fail(TSDF_ERR_HIP, "statistics reduction failed: %s", hipGetErrorString(e));
+  }
   if (!r) r = tsdf_publish(m, sensor, pos, data, capacity, count);
   if (!r) r = tsdf_gc(m, sensor, nullptr);
   e = hipStreamSynchronize(E->stream);
   if (!r && e != hipSuccess) r = fail(TSDF_ERR_HIP, "stream synchronisation failed: %s", hipGetErrorString(e));
+  if (!r) {
+    // publish read the engine counters after the reduction above
+    const unsigned long long* st = E->ctr_host->stats;
+    if (st[ST_INSERT_FAIL] || st[ST_HEAP_FAIL] || st[ST_UPD_DROPPED])
+      r = fail(TSDF_ERR_DROPPED, "the scan dropped %llu updates (%llu block inserts refused, %llu heap blocks refused)",
+               st[ST_UPD_DROPPED], st[ST_INSERT_FAIL], st[ST_HEAP_FAIL]);
+  }
   return r;
 }
 
@@ -1063,6 +1086,9 @@ int tsdf_last_stats(tsdf_engine* E, tsdf_scan_stats* out) {
   out->published = (int64_t)s[ST_PUBLISHED];
   out->gc_removed = (int64_t)s[ST_GC_REMOVED];
   out->quota_failures = (int64_t)s[ST_QUOTA_FAIL];
+  if (out->insert_failures || out->heap_failures || out->updates_dropped)
+    return fail(TSDF_ERR_DROPPED, "the last pass dropped %lld updates (%lld block inserts refused, %lld heap blocks refused)",
+                (long long)out->updates_dropped, (long long)out->insert_failures, (long long)out->heap_failures);
   return TSDF_OK;
 }
 
@@ -1172,11 +1198,13 @@ int tsdf_batch_graph_create(tsdf_engine* E, tsdf_map* const* maps, int32_t nmaps
   g->stride = 1;
   for (int i = 0; i < nmaps; ++i) {
     int r = check_map(maps[i]);
+    if (!r && maps[i]->eng != E) r = fail(TSDF_ERR_INVALID_ARG, "map %d belongs to another engine", i);
     if (r) {
       delete g;
       return r;
     }
-    g->maps.push_back(maps[i]);
+    g->slots.push_back(maps[i]->slot);
+    g->ids.push_back(maps[i]->id);
     g->stride = std::max(g->stride, maps[i]->desc.max_vox);
   }
   int r = graph_capture(g);
@@ -1198,13 +1226,16 @@ int tsdf_batch_graph_launch(tsdf_batch_graph* g, const tsdf_scan_desc* scans, in
   int64_t P = 0;
   for (int i = 0; i < nscans; ++i) {
     const tsdf_scan_desc& s = scans[i];
-    if (s.map_index < 0 || s.map_index >= (int)g->maps.size()) return fail(TSDF_ERR_INVALID_ARG, "map_index out of range");
+    if (s.map_index < 0 || s.map_index >= (int)g->slots.size()) return fail(TSDF_ERR_INVALID_ARG, "map_index out of range");
+    const int slot = g->slots[s.map_index];
+    if (!E->map_by_slot[slot] || E->map_by_slot[slot]->id != g->ids[s.map_index])
+      return fail(TSDF_ERR_INVALID_ARG, "scan %d: map %d of the graph was destroyed", i, s.map_index);
     if (s.n_points < 0 || s.point_offset < 0) return fail(TSDF_ERR_INVALID_ARG, "scan %d: negative size/offset", i);
     if (s.point_offset != (nscans ? scans[0].point_offset : 0) + P)
       return fail(TSDF_ERR_INVALID_ARG, "scans of a batch must be contiguous and in order in the point buffer");
     for (int k = 0; k < 3; ++k)
       if (!std::isfinite(s.sensor[k])) return fail(TSDF_ERR_INVALID_ARG, "non-finite sensor position");
-    sd[i].map = g->maps[s.map_index]->slot;
+    sd[i].map = slot;
     sd[i].n = s.n_points;
     sd[i].off = (int32_t)P;
     sd[i].s[0] = s.sensor[0];

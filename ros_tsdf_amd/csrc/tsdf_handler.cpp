@@ -114,6 +114,12 @@ void TSDFHandler::processPointCloudAndUpdateVoxels(pcl::PointCloud<pcl::PointXYZ
   float* pos = cap ? publish_voxels_pos_h[0].data() : nullptr;
   float* data = cap ? &publish_voxels_data_h[0].sdf : nullptr;
   int r = tsdf_process_scan(m, pts, n, sensor, pos, data, cap, &count);
+  // the reference prints on heap overflow and carries on (tsdf_handler.cu:43-45): so does the
+  // handler when blocks were refused; the published voxels are complete
+  if (r == TSDF_ERR_DROPPED) {
+    std::fprintf(stderr, "tsdf: %s\n", tsdf_last_error());
+    r = TSDF_OK;
+  }
   if (r != TSDF_OK) {
     report("processPointCloudAndUpdateVoxels");
     if (publish_voxels_size_h) *publish_voxels_size_h = 0;
@@ -135,8 +141,12 @@ void TSDFHandler::allocateVoxelBlocksAndUpdateVoxels(pcl::PointXYZ* lidar_points
     return;
   }
   if (tsdf_integrate(m, reinterpret_cast<const float*>(lidar_points_d), point_cloud_size_h, s) != TSDF_OK ||
-      tsdf_engine_synchronize(engine()) != TSDF_OK)
+      tsdf_engine_synchronize(engine()) != TSDF_OK) {
     report("allocateVoxelBlocksAndUpdateVoxels");
+    return;
+  }
+  tsdf_scan_stats st;
+  if (tsdf_last_stats(engine(), &st) == TSDF_ERR_DROPPED) std::fprintf(stderr, "tsdf: %s\n", tsdf_last_error());
 }
 
 void TSDFHandler::getVoxelBlocks(int, pcl::PointXYZ*, Vector3f*, int*, Vector3f*, int*) {}

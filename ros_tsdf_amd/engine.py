@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 import weakref
 
 import numpy as np
@@ -131,8 +132,13 @@ class Engine:
         return Map(self, params, buckets_log2, overflow_buckets, max_chain, block_quota, admit)
 
     def stats(self) -> dict:
+        """Counters of the last integration pass (and of the last publish / gc).  A pass that dropped
+        updates shows them in insert_failures / heap_failures / updates_dropped (the C ABI's
+        TSDF_ERR_DROPPED status, which is not an error of this query)."""
         s = abi.ScanStats()
-        check(abi.load().tsdf_last_stats(self.h, C.byref(s)))
+        r = abi.load().tsdf_last_stats(self.h, C.byref(s))
+        if r != abi.TSDF_ERR_DROPPED:
+            check(r)
         return s.as_dict()
 
     def integrate_batch(self, maps, scans, points: DeviceBuffer):
@@ -302,6 +308,8 @@ class Map:
         float4 points, publish into (pos, data), gc.  out: (pos, data) arrays kept by the caller, as in
         publish; otherwise new arrays of capacity rows (default 2^20)."""
         pts = np.ascontiguousarray(points, np.float32)
+        if pts.ndim != 2 or pts.shape[1] != 4:
+            raise ValueError("points must be float4 rows (x, y, z, pad) of shape (n, 4), got %s" % (pts.shape,))
         s = _vec3(sensor)
         n = C.c_int64()
         if out is not None:
@@ -311,8 +319,13 @@ class Map:
                 capacity = 1 << 20
             pos = np.zeros((capacity, 3), np.float32)
             data = np.zeros((capacity, 2), np.float32)
-        check(abi.load().tsdf_process_scan(self.h, _fptr(pts), pts.shape[0], s, _fptr(pos), _fptr(data),
-                                           capacity, C.byref(n)))
+        r = abi.load().tsdf_process_scan(self.h, _fptr(pts), pts.shape[0], s, _fptr(pos), _fptr(data),
+                                         capacity, C.byref(n))
+        if r == abi.TSDF_ERR_DROPPED:
+            # the scan ran to the end and the outputs are complete; the map lost the refused blocks' updates
+            warnings.warn(abi.load().tsdf_last_error().decode(), abi.DroppedUpdatesWarning, stacklevel=2)
+        else:
+            check(r)
         k = min(n.value, capacity)
         return pos[:k], data[:k], n.value
 

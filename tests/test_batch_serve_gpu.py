@@ -340,3 +340,40 @@ def test_autotune_knobs_do_not_change_results():
         assert best["traverse_block"] in (128, 256) and best["buckets_log2"] in (14, 15)
         assert e.get_launch()["traverse_block"] == best["traverse_block"]
         buf.free()
+
+
+def test_graph_launch_after_a_map_is_destroyed(eng):
+    """A graph does not keep its maps alive: after one of them is closed (and its slot reused by a new map),
+    a launch with a scan of it is refused with nothing enqueued, while launches with scans of the other map
+    still run.  A map of another engine is refused when the graph is created."""
+    import ctypes as C
+    from ros_tsdf_amd import BatchGraph, Engine
+    from ros_tsdf_amd.engine import _scan_array
+    scans = small_scans(2, 1)
+    pool = workloads.Pool(scans)
+    buf = eng.buffer(pool.points.nbytes)
+    buf.upload(pool.points)
+    m0, m1 = eng.map(P, buckets_log2=12), eng.map(P, buckets_log2=12)
+    g = BatchGraph(eng, [m0, m1], max_points=pool.total, max_scans=2)
+    both = [(pool.maps[i], pool.counts[i], pool.offsets[i], pool.sensors[i]) for i in range(len(pool))]
+    m1.close()
+    newcomer = eng.map(P, buckets_log2=12)  # may take the closed map's slot
+    L = abi.load()
+    arr = _scan_array(both)
+    assert L.tsdf_batch_graph_launch(C.c_void_p(g.h), arr, 2, C.c_void_p(buf.ptr)) == abi.TSDF_ERR_INVALID_ARG
+    assert "destroyed" in L.tsdf_last_error().decode()
+    eng.synchronize()
+    assert m0.num_blocks() == 0 and newcomer.num_blocks() == 0
+    g.launch(both[:1], buf)  # map 0 alone
+    eng.synchronize()
+    assert m0.num_blocks() > 0 and newcomer.num_blocks() == 0
+    g.close()
+    with Engine(heap_blocks=1 << 12, total_buckets=1 << 12) as other:
+        om = other.map(P, buckets_log2=8)
+        with pytest.raises(abi.TsdfError) as ei:
+            BatchGraph(eng, [m0, om], max_points=pool.total, max_scans=2)
+        assert ei.value.code == abi.TSDF_ERR_INVALID_ARG
+        om.close()
+    m0.close()
+    newcomer.close()
+    buf.free()

@@ -532,3 +532,83 @@ def test_fold_tiers_at_their_bounds(eng, oracle_lib, n, cols):
     compare_maps(m, om, TIER_PARAMS[1], *tol(TIER_PARAMS))
     m.close()
     buf.free()
+
+
+def test_block_beyond_every_range_folds_in_point_rounds(eng, oracle_lib):
+    """270,000 records in one block in one pass, 2,812 in each of its 96 updated voxels: no voxel range fits
+    (a voxel holds more than 2,048 records), so the block is folded in rounds of consecutive points.  Nothing
+    is dropped and every voxel equals the oracle's."""
+    scans = tier_scans(270000, cols=16)
+    pool = workloads.Pool([(p, s, 0) for p, s in scans])
+    buf = eng.buffer(pool.points.nbytes)
+    buf.upload(pool.points)
+    m = eng.map(TIER_PARAMS, buckets_log2=10)
+    eng.integrate_batch([m], [(0, pool.counts[i], pool.offsets[i], pool.sensors[i]) for i in range(len(pool))], buf)
+    eng.synchronize()
+    st = eng.stats()
+    assert st["voxel_updates"] == 6 * pool.points.shape[0] == 270000
+    assert st["updates_dropped"] == 0 and st["unique_voxels"] == 6 * 16
+    om = oracle_lib.OracleMap(TIER_PARAMS)
+    for p, s in scans:
+        om.integrate(p, s)
+    k, _, w = m.export()
+    i10 = [j for j, kk in enumerate(k.tolist()) if kk == [10, 0, 0]]
+    assert len(i10) == 1 and (w[i10[0]] != 0).sum() == 96
+    compare_maps(m, om, TIER_PARAMS[1], *tol(TIER_PARAMS))
+    m.close()
+    om.close()
+    buf.free()
+
+
+def last_stats_status(e):
+    import ctypes as C
+    st = abi.ScanStats()
+    return abi.load().tsdf_last_stats(C.c_void_p(e.h), C.byref(st)), st.as_dict()
+
+
+def test_dropped_updates_are_a_status(oracle_lib):
+    """Blocks refused for lack of heap make tsdf_last_stats and tsdf_process_scan return TSDF_ERR_DROPPED
+    (with the counts filled in and the outputs complete); a pass without refusals returns TSDF_OK."""
+    from ros_tsdf_amd import Engine
+    with Engine(heap_blocks=64, total_buckets=1 << 12, max_batch_points=1 << 14) as e:
+        pts, s = dense_scan(0, n=4096)
+        m = e.map(D, buckets_log2=8)
+        buf = e.buffer(pts.nbytes)
+        buf.upload(pts)
+        m.integrate(buf, pts.shape[0], s)
+        code, st = last_stats_status(e)
+        assert code == abi.TSDF_ERR_DROPPED and st["heap_failures"] > 0 and st["updates_dropped"] > 0
+        assert "dropped" in abi.load().tsdf_last_error().decode()
+        # the full per-scan call on a second map: the status, and a publish that still ran
+        m2 = e.map(D, buckets_log2=8)
+        with pytest.warns(abi.DroppedUpdatesWarning):
+            pos, data, n = m2.process_scan(pts, s)
+        assert n == pos.shape[0] == 0  # the first map holds every heap block
+        m.gc((1e4, 1e4, 0.0))          # frees them all
+        with pytest.warns(abi.DroppedUpdatesWarning):
+            m2.process_scan(pts, s)
+        assert m2.num_blocks() == 64
+        pos, data, n = m2.publish(s)
+        assert n > 0
+        m2.gc((1e4, 1e4, 0.0))
+        small, s2 = dense_scan(1, n=16)
+        buf.upload(small)
+        m2.integrate(buf, small.shape[0], s2)
+        code, st = last_stats_status(e)
+        assert code == abi.TSDF_OK and st["updates_dropped"] == 0
+        m.close()
+        m2.close()
+        buf.free()
+
+
+def test_process_scan_refuses_rows_that_are_not_float4(eng):
+    """Host points must be float4 rows (x, y, z, pad): an (n, 3) array would be read as 16-byte rows."""
+    pts, s = dense_scan(0, n=64)
+    m = eng.map(D, buckets_log2=8)
+    for bad in (pts[:, :3], pts.reshape(-1), pts[:, :2]):
+        with pytest.raises(ValueError):
+            m.process_scan(bad, s)
+    assert m.num_blocks() == 0
+    m.process_scan(pts, s)
+    assert m.num_blocks() > 0
+    m.close()
