@@ -7,7 +7,9 @@ Properties checked (none depends on the map's size):
 - evicting (export, destroy) returns the blocks to the heap, restoring takes them again;
 - packed state formats re-encode exported values to the same values;
 - a block quota refuses the excess blocks, which leave no key behind;
-- importing a map's own snapshot over it changes nothing.
+- importing a map's own snapshot over it changes nothing;
+- a map evicted, restored into another table shape and integrated further equals the FP64 oracle fed
+  every scan, in all three state formats.
 """
 import numpy as np
 import pytest
@@ -135,3 +137,57 @@ def test_import_argument_errors(scans):
         assert np.array_equal(k, keys)
         assert (s == 0.25).all() and (w == 2.0).all()
         m.close()
+
+
+# Tolerances of each state format against the FP64 oracle after SCANS passes: max sdf error in units of T,
+# and the weight bound as (relative, absolute).  FP16 sdf/T is rounded once per pass, by at most half an
+# ulp of a value below 2 in magnitude (2^-11), and the fold never amplifies an earlier rounding (the old
+# sdf is weighted by W / (W + w) < 1): SCANS * 2^-11.  A weight's rounding is not damped (W + w carries it
+# on), so the fixed-point weight may be off by half a step (max_weight / 65535) per pass: SCANS half steps.
+# FP16 weights keep 11 significant bits (2^-12 relative per pass, bounded here by 1e-2), and the
+# fixed-point format's sdf bound of 1e-2 T is that of the packed-state error test in
+# tests/test_batch_serve_gpu.py.
+FORMAT_TOL = {
+    abi.TSDF_STATE_F32: (configs.SDF_TOL_T, (0.0, configs.WEIGHT_TOL_REL * configs.DENSE_PARAMS[2])),
+    abi.TSDF_STATE_F16: (SCANS * 2.0 ** -11, (1e-2, 0.0)),
+    abi.TSDF_STATE_F16_QW: (1e-2, (0.0, SCANS * 0.5 * configs.DENSE_PARAMS[2] / 65535 + 1e-4)),
+}
+
+
+@pytest.mark.parametrize("fmt", [abi.TSDF_STATE_F32, abi.TSDF_STATE_F16, abi.TSDF_STATE_F16_QW])
+def test_evict_restore_continue_matches_oracle(scans, fmt, oracle_lib):
+    """Three scans, export, destroy (evict), import into a map of another table shape, two more scans: the
+    map equals an oracle map fed all five scans (keys and touched voxels exact, values within the format's
+    tolerance), and so does its publish."""
+    params = configs.DENSE_PARAMS
+    T = params[1]
+    sdf_tol, (w_rel, w_abs) = FORMAT_TOL[fmt]
+    with Engine(heap_blocks=1 << 18, total_buckets=1 << 18, state_format=fmt) as eng:
+        om = oracle_lib.OracleMap(params)
+        for p, s in scans:
+            om.integrate(p, s, threads=8)
+        a = eng.map(params, buckets_log2=15)
+        integrate(eng, a, scans[:3])
+        snap = a.export()
+        a.close()
+        b = eng.map(params, buckets_log2=13, max_chain=16)
+        assert b.import_blocks(*snap) == (snap[0].shape[0], 0)
+        integrate(eng, b, scans[3:])
+        gk, gs, gw = b.export()
+        ok, os_, ow = om.export()
+        assert np.array_equal(gk, ok)
+        t = ow != 0
+        assert np.array_equal(gw != 0, t)
+        e = np.abs(gs[t].astype(np.float64) - os_[t]) / T
+        we = np.abs(gw[t].astype(np.float64) - ow[t])
+        assert e.max() <= sdf_tol, e.max()
+        assert (we <= w_rel * ow[t] + w_abs).all(), we.max()
+        s = scans[-1][1]
+        gpos, gdata, n = b.publish(s)
+        opos, osdf, ow_p = om.publish(s)
+        assert n == opos.shape[0] > 0
+        go, oo = np.lexsort(gpos.T[::-1]), np.lexsort(opos.T[::-1])
+        assert np.array_equal(gpos[go].view(np.uint32), opos[oo].view(np.uint32))
+        assert np.abs(gdata[go, 0].astype(np.float64) - osdf[oo]).max() <= sdf_tol * T
+        b.close()
+        om.close()
