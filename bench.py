@@ -16,7 +16,7 @@ Also reported on the same JSON line:
   serving      Poisson arrivals over 16 maps at 400 scans/s: continuous
                batching vs plain per-scan serving, p50/p99 latency
   roofline     the dominant kernel against MI355X HBM bandwidth
-  cpu_baseline the oracle's multithreaded CPU path on a bounded sample
+  cpu_baseline the oracle's multithreaded CPU path over the whole dense workload
 """
 from __future__ import annotations
 
@@ -122,9 +122,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--only", default="all", choices=["all", "plain", "batched_full"],
                     help="plain: the headline only; batched_full: only the full-size batched comparison (profiling)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--error-scans", type=int, default=200,
-                    help="dense scans integrated before the FP16/FP32 error figures (also the CPU-baseline sample)")
+                    help="dense scans integrated before the FP16/FP32 error figures (also the CPU-baseline sample); "
+                         "all 200 of the workload by default, whatever --steps is")
     ap.add_argument("--serve-requests", type=int, default=2000)
     args = ap.parse_args()
 
@@ -149,11 +149,13 @@ def main():
 
     K, W = args.steps, args.warmup
     n_dense = configs.DENSE.n_scans
-    t_gen = time.perf_counter()
-    need = min(n_dense, K + W)
-    dpts, dsens, fills = workloads.generate(configs.DENSE, range(need))
-    dense = [(dpts[k], dsens[k], 0) for k in range(need)]
     extras = args.only == "all"
+    t_gen = time.perf_counter()
+    # the timed scans cycle through the first K + W; the error / CPU leg integrates all 200 whatever K is
+    need = min(n_dense, K + W)
+    n_gen = n_dense if extras else need
+    dpts, dsens, fills = workloads.generate(configs.DENSE, range(n_gen))
+    dense = [(dpts[k], dsens[k], 0) for k in range(n_gen)]
     if extras:
         bmaps_pts, bmaps_sens = workloads.generate_maps(configs.DENSE, configs.BATCH_MAPS, 16)
         smaps_pts, smaps_sens = workloads.generate_maps(configs.DENSE, configs.SERVE_MAPS, 24)
@@ -204,6 +206,8 @@ def main():
             traffic = tj.get(dom_kernel)
         except Exception:
             traffic = None
+    traffic_source = ("profiles/traffic_dense.json: rocprofv3 FETCH_SIZE and WRITE_SIZE passes over this workload, "
+                      "scaled by the copy-kernel factors; a stored profile, not counted in this run") if traffic else None
     # the same kernel against the VALU issue rate: its instruction count per launch (rocprofv3
     # SQ_INSTS_VALU, profiles/valu_dense.json) over its live duration
     valu = None
@@ -223,7 +227,7 @@ def main():
         except Exception:
             valu = None
     roofline = {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
-                "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
+                "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic, "traffic_source": traffic_source,
                 "alg_bytes_per_launch": alg_bytes, "bytes_per_point": float(np.mean(bpt)),
                 "pass_achieved_GBps": alg_bytes / (stage_ms["total"] * 1e-3) / 1e9, "valu": valu}
     log("plain: %.3e pts/s, %.3f ms/scan, stages %s" % (value, ms_per_step, {k: round(v, 4) for k, v in stage_ms.items()}))
@@ -369,16 +373,18 @@ def main():
             mp.close()
             packed[name] = {"engine": e, "buf": pb, "map": e.map(configs.DENSE_PARAMS, buckets_log2=17),
                             "result": {"points_per_s": hn / hs, "bytes_per_voxel": 4, "bytes_per_voxel_update_rw": 8}}
-        # error after the first E dense scans, every format, against the oracle (whose multithreaded run
-        # over the same scans is also the CPU baseline)
-        E = min(args.error_scans, need)
+        # error after the dense workload's scans (all 200 by default), every format, against the oracle, whose
+        # multithreaded run over the same scans is also the CPU baseline
+        E = min(args.error_scans, n_gen)
         import oracle
         om = oracle.OracleMap(configs.DENSE_PARAMS)
         eng32 = Engine(heap_blocks=1 << 20, total_buckets=1 << 21, max_batch_points=1 << 17)
         m32 = eng32.map(configs.DENSE_PARAMS, buckets_log2=17)
         buf32 = eng32.buffer(dpool.points.nbytes)
         buf32.upload(dpool.points)
-        nthreads = min(16, os.cpu_count() or 1)
+        # every core this process may run on (nproc), one oracle thread each
+        usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        nthreads = max(1, usable)
         tcpu = 0.0
         cpu_pts = 0
         cpu_scans = 0
@@ -388,11 +394,9 @@ def main():
             m32.integrate(buf32, dpool.counts[k], dpool.sensors[k], offset_points=dpool.offsets[k])
             t0 = time.perf_counter()
             om.integrate(dense[k][0], dense[k][1], threads=nthreads)
-            dt = time.perf_counter() - t0
-            if tcpu < args.cpu_seconds:
-                tcpu += dt
-                cpu_pts += dpool.counts[k]
-                cpu_scans += 1
+            tcpu += time.perf_counter() - t0
+            cpu_pts += dpool.counts[k]
+            cpu_scans += 1
         ok, os_, ow = om.export()
         T = configs.DENSE_PARAMS[1]
         errs = {}
@@ -412,8 +416,9 @@ def main():
         result["fp32_error_vs_oracle"] = errs["fp32"]
         log("errors", errs)
         cpu_baseline = {"value": cpu_pts / tcpu, "unit": "points/s", "cores": nthreads, "kind": "port",
-                        "sample": "oracle multithreaded path, first %d dense scans (%d points), %.1f s" % (
-                            cpu_scans, cpu_pts, tcpu)}
+                        "nproc": usable, "os_cpu_count": os.cpu_count(), "threads": nthreads,
+                        "sample": "oracle multithreaded path (std::thread, one per core), the whole dense workload: "
+                                  "%d scans, %d points, %.1f s" % (cpu_scans, cpu_pts, tcpu)}
         om.close()
         m32.close()
         buf32.free()
