@@ -144,8 +144,7 @@ struct tsdf_engine {
   uint8_t* new_map = nullptr;
   int4* fold_desc = nullptr;
   unsigned long long* part = nullptr;
-  int64_t fold_cap = 0;
-  int64_t fold_base[8] = {};
+  int64_t fold_cap = 0;  // fold descriptors: one per touched block of a pass
   uint8_t* touched_map = nullptr;
   uint32_t* sel = nullptr;
   uint32_t* pub_cnt = nullptr;
@@ -210,8 +209,6 @@ struct tsdf_engine {
     D.new_map = new_map;
     D.fold_desc = fold_desc;
     D.part = part;
-    D.fold_cap = fold_cap;
-    for (int c = 0; c < 8; ++c) D.fold_base[c] = fold_base[c];
     D.touched_map = touched_map;
     D.seg = seg;
     D.seg_cap = seg_cap;
@@ -323,14 +320,9 @@ int ensure_scratch(tsdf_engine* E, int64_t P, int stride) {
   CHK(dev_alloc(&E->seg, np * ns));
   (void)hipFree(E->fold_desc);
   E->fold_desc = nullptr;
-  // one region per size class, each as large as that class can get
+  // a pass touches at most one block per record slot, and at most every entry
   E->fold_cap = std::min<int64_t>(E->entries, np * ns);
-  int64_t fold_total = 0;
-  for (int c = 0; c < FC_COUNT; ++c) {
-    E->fold_base[c] = fold_total;
-    fold_total += fold_class_cap(c, E->fold_cap, np * ns);
-  }
-  CHK(dev_alloc(&E->fold_desc, fold_total));
+  CHK(dev_alloc(&E->fold_desc, E->fold_cap));
   E->pcap = np;
   E->stride_cap = ns;
   E->seg_cap = np * ns;
@@ -402,10 +394,11 @@ int stage_descs(tsdf_engine* E, const PassHdr& hdr, const ScanDev* scans, int ns
 }
 
 // Enqueues the pass kernels sized for P points (grid sizes only; the
-// kernels read the actual counts from device memory).  inl: the pass's
-// descriptors as kernel arguments, or nscans = -1 when they were staged.
-int enqueue_pass(tsdf_engine* E, int64_t P, int tid, const PassInline& inl) {
-  Dev D = E->view();
+// kernels read the actual counts from device memory) over the pass scratch
+// of D, whose pair rows and records have `stride` slots per point.  inl: the
+// pass's descriptors as kernel arguments, or nscans = -1 when they were
+// staged.
+int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int tid, const PassInline& inl) {
   hipStream_t st = E->stream;
   const tsdf_launch_config& L = E->launch;
   hipLaunchKernelGGL(k_pass_begin, dim3(64), dim3(256), 0, st, D, kPassStats, inl);
@@ -415,7 +408,7 @@ int enqueue_pass(tsdf_engine* E, int64_t P, int tid, const PassInline& inl) {
   CHK(timing_mark(E, tid, 1));
   // (heap assignment runs in k_pass_scan, inside the scan stage: the assign stage is empty)
   CHK(timing_mark(E, tid, 2));
-  const int64_t tmax = std::min<int64_t>(E->entries, P * E->stride_cap);
+  const int64_t tmax = std::min<int64_t>(E->entries, P * stride);
   // tiles of touched blocks are taken grid-stride by at most 256 workgroups
   // (the touched count is only known on the device; most passes have tens)
   const int sg = grid_for(tmax, kScanTile, 256);
@@ -499,7 +492,7 @@ int run_batch(tsdf_engine* E, tsdf_map* const* maps, int nmaps, const tsdf_scan_
   }
   int tid;
   CHK(timing_begin(E, 0, &tid));
-  CHK(enqueue_pass(E, P, tid, inl));
+  CHK(enqueue_pass(E, E->view(), P, E->stride_cap, tid, inl));
   timing_end(E, tid);
   E->last_points = P;
   return TSDF_OK;
@@ -1170,7 +1163,7 @@ static int graph_capture(tsdf_batch_graph* g) {
   HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
   PassInline staged{};
   staged.nscans = -1;  // a replay's descriptors are copied in before each launch
-  int r = enqueue_pass(E, g->max_points, -1, staged);
+  int r = enqueue_pass(E, E->view(), g->max_points, E->stride_cap, -1, staged);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(E->stream, &graph);
   E->timing = t;
