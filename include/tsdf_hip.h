@@ -160,6 +160,23 @@ int tsdf_map_import(tsdf_map* map, const int32_t* keys, const float* voxels, int
  * point buffer must stay valid until the work completes. */
 int tsdf_integrate(tsdf_map* map, const float* points_dev, int64_t n, const float sensor[3]);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tsdf_integrate with the pass scratch taken from a fixed
+ * 96 MiB device array instead of the engine's scratch, which tsdf_integrate
+ * grows with hipMalloc when a scan outgrows it.  The call allocates and frees
+ * nothing.  The array holds the pair rows and update records of n points at
+ * the map's MAX_VOXELS_TRAVERSED slots each, plus one fold descriptor per
+ * entry of the map (16 bytes per point and slot, 16 per descriptor, 1 per
+ * point): a 131,072-point scan at voxel 0.1 m / truncation 0.4 m (32 slots)
+ * of a map of 2^17 home buckets needs 84 MiB.  Larger calls return
+ * TSDF_ERR_CAPACITY with nothing enqueued.  Each GPU holds one such array
+ * for the life of the process, shared by every engine on it: calls on one
+ * GPU must not overlap (on one engine they are ordered by its stream; calls
+ * from engines with different streams on one GPU race unless the caller
+ * orders them).  Results equal tsdf_integrate's.  Asynchronous. */
+int tsdf_integrate_noalloc(tsdf_map* map, const float* points_dev, int64_t n, const float sensor[3]);
+
 /* Replaces: TSDFHandler::retrievePublishableVoxels (H:1193-1224).  Writes
  * every voxel with weight != 0 in blocks whose centre lies within
  * sqrt(publish_distance_squared) of the sensor: positions float[n][3] and

@@ -112,6 +112,7 @@ SIGNATURES = {
     "tsdf_map_export": [vp, i32p, fp, C.c_int64, i64p],
     "tsdf_map_import": [vp, i32p, fp, C.c_int64, i64p, i64p],
     "tsdf_integrate": [vp, vp, C.c_int64, fp],
+    "tsdf_integrate_noalloc": [vp, vp, C.c_int64, fp],
     "tsdf_publish": [vp, fp, fp, fp, C.c_int64, i64p],
     "tsdf_publish_msg": [vp, fp, fp, C.c_int64, i64p],
     "tsdf_gc": [vp, fp, i64p],

@@ -92,6 +92,51 @@ auto with_state(int fmt, F&& f) {
 
 }  // namespace
 
+// ---- pass scratch without allocation -------------------------------------------------------
+// tsdf_integrate_noalloc takes its pass scratch (pair rows, run counts,
+// update records, fold descriptors) from this file-scope device array: each
+// GPU that loads the library holds one copy until the process ends, and
+// every call on that GPU shares it.  Nothing in it needs to start at zero:
+// a pass writes every pair row, run count, record and descriptor it reads.
+// In host code the name stands for a host-side shadow, never a device
+// address: each device's address is resolved once with hipGetSymbolAddress
+// (with that device current, when the first engine is created on it) and
+// only that address is passed to kernels.
+constexpr int64_t kNoAllocScratchBytes = int64_t(96) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_resolved[kMaxDevices] = {};
+void* g_scratch_addr[kMaxDevices] = {};
+hipError_t g_scratch_status[kMaxDevices] = {};
+
+// Resolves the current device's copy of scratch (once per device).
+void resolve_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_resolved[dev]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) (void)hipGetLastError();
+  g_scratch_status[dev] = e;
+  g_scratch_addr[dev] = e == hipSuccess ? p : nullptr;
+  g_scratch_resolved[dev] = true;
+}
+
+// The device address of device dev's scratch, or null (with the reason in *err).
+void* scratch_address(int dev, hipError_t* err) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch_resolved[dev]) {
+    *err = hipErrorNotInitialized;
+    return nullptr;
+  }
+  *err = g_scratch_status[dev];
+  return g_scratch_addr[dev];
+}
+}  // namespace
+
 struct tsdf_batch_graph;
 
 struct tsdf_map {
@@ -732,6 +777,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
     return r;
   };
   if (hipSetDevice(E->device) != hipSuccess) return cleanup(fail(TSDF_ERR_HIP, "hipSetDevice failed"));
+  resolve_scratch(E->device);  // for tsdf_integrate_noalloc; a failure is reported by that call
   if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(TSDF_ERR_HIP, "hipStreamCreate failed"));
   int r;
@@ -987,6 +1033,65 @@ int tsdf_integrate_batch(tsdf_engine* E, tsdf_map* const* maps, int32_t nmaps, c
                          const float* points_dev) {
   if (!E) return fail(TSDF_ERR_INVALID_ARG, "null engine");
   return run_batch(E, maps, nmaps, scans, nscans, points_dev);
+}
+
+int tsdf_integrate_noalloc(tsdf_map* m, const float* points_dev, int64_t n, const float sensor[3]) {
+  CHK(check_map(m));
+  if (!sensor) return fail(TSDF_ERR_INVALID_ARG, "null sensor");
+  if (n < 0 || n > kMaxPassPoints) return fail(TSDF_ERR_CAPACITY, "point count %lld out of range", (long long)n);
+  if (n > 0 && !points_dev) return fail(TSDF_ERR_INVALID_ARG, "null point buffer");
+  if (n > 0 && reinterpret_cast<uintptr_t>(points_dev) % alignof(float4) != 0)
+    return fail(TSDF_ERR_INVALID_ARG, "the point buffer must be 16-byte aligned (float4 rows)");
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(sensor[k])) return fail(TSDF_ERR_INVALID_ARG, "non-finite sensor position");
+  tsdf_engine* E = m->eng;
+  // the pass scratch layout, as ensure_scratch lays it out for the engine's
+  // own scratch: pair rows and records with max_vox slots per point, one fold
+  // descriptor per block the pass can touch (at most the map's entries)
+  const int64_t pc = std::max<int64_t>(n, 1);
+  const int64_t stride = m->desc.max_vox;
+  const int64_t slots = pc * stride;
+  const int64_t fold_cap = std::min<int64_t>(((int64_t)m->desc.n_home + m->desc.n_ovf) * kBucket, slots);
+  auto up = [](int64_t b) { return (b + 255) & ~int64_t(255); };
+  const int64_t o_pe = 0;
+  const int64_t o_pr = up(o_pe + (int64_t)sizeof(uint32_t) * slots);
+  const int64_t o_seg = up(o_pr + (int64_t)sizeof(uint32_t) * slots);
+  const int64_t o_fd = up(o_seg + (int64_t)sizeof(unsigned long long) * slots);
+  const int64_t o_pn = up(o_fd + (int64_t)sizeof(int4) * fold_cap);
+  const int64_t bytes = up(o_pn + pc);
+  if (bytes > kNoAllocScratchBytes)
+    return fail(TSDF_ERR_CAPACITY,
+                "tsdf_integrate_noalloc: %lld points at %lld voxels per point need %lld bytes of pass scratch, "
+                "more than its fixed %lld",
+                (long long)n, (long long)stride, (long long)bytes, (long long)kNoAllocScratchBytes);
+  hipError_t se = hipSuccess;
+  unsigned char* base = static_cast<unsigned char*>(scratch_address(E->device, &se));
+  if (!base) return fail(TSDF_ERR_HIP, "the pass scratch of device %d is unavailable: %s", E->device, hipGetErrorString(se));
+  CHK(set_device(E));
+  Dev D = E->view();
+  D.pair_e = reinterpret_cast<uint32_t*>(base + o_pe);
+  D.pair_r = reinterpret_cast<uint32_t*>(base + o_pr);
+  D.seg = reinterpret_cast<unsigned long long*>(base + o_seg);
+  D.fold_desc = reinterpret_cast<int4*>(base + o_fd);
+  D.pair_n = base + o_pn;
+  D.pcap = pc;
+  D.seg_cap = slots;
+  PassInline inl{};
+  inl.hdr.points = reinterpret_cast<const float4*>(points_dev);
+  inl.hdr.total = (int32_t)n;
+  inl.hdr.nscans = 1;
+  inl.hdr.stride = (int32_t)stride;
+  inl.nscans = 1;
+  inl.scans[0].map = m->slot;
+  inl.scans[0].n = (int32_t)n;
+  inl.scans[0].off = 0;
+  for (int k = 0; k < 3; ++k) inl.scans[0].s[k] = sensor[k];
+  int tid;
+  CHK(timing_begin(E, 0, &tid));
+  CHK(enqueue_pass(E, D, n, stride, tid, inl));
+  timing_end(E, tid);
+  E->last_points = n;
+  return TSDF_OK;
 }
 
 int tsdf_publish(tsdf_map* m, const float sensor[3], float* pos, float* data, int64_t capacity, int64_t* count) {

@@ -233,6 +233,12 @@ class Map:
             points.check_points(offset_points, n)  # raises with the reason
         check(abi.load().tsdf_integrate(self.h, points.ptr + 16 * offset_points, n, _vec3(sensor)))
 
+    def integrate_noalloc(self, points: DeviceBuffer, n: int, sensor, offset_points: int = 0):
+        """integrate() through tsdf_integrate_noalloc: the pass scratch comes from the library's fixed
+        per-GPU array, so the call allocates nothing (TsdfError TSDF_ERR_CAPACITY beyond its size)."""
+        points.check_points(offset_points, n)
+        check(abi.load().tsdf_integrate_noalloc(self.h, points.ptr + 16 * offset_points, n, _vec3(sensor)))
+
     def publish(self, sensor, capacity=None, out=None):
         """Voxels to publish: (pos float32[k,3], data float32[k,2] {sdf, weight}, total).  k = min(total,
         capacity).  out: (pos, data) float32 arrays of c rows to fill instead of new ones (capacity defaults
