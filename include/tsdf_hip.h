@@ -166,10 +166,11 @@ int tsdf_integrate(tsdf_map* map, const float* points_dev, int64_t n, const floa
  * 96 MiB device array instead of the engine's scratch, which tsdf_integrate
  * grows with hipMalloc when a scan outgrows it.  The call allocates and frees
  * nothing.  The array holds the pair rows and update records of n points at
- * the map's MAX_VOXELS_TRAVERSED slots each, plus one fold descriptor per
- * entry of the map (16 bytes per point and slot, 16 per descriptor, 1 per
- * point): a 131,072-point scan at voxel 0.1 m / truncation 0.4 m (32 slots)
- * of a map of 2^17 home buckets needs 84 MiB.  Larger calls return
+ * the map's MAX_VOXELS_TRAVERSED slots each (16 bytes per point and slot, 1
+ * per point) and the fold descriptors of each block size class, 16 bytes
+ * each, as many as the class can hold in a pass over the map: a 131,072-point
+ * scan at voxel 0.1 m / truncation 0.4 m (32 slots) of a map of 2^17 home
+ * buckets needs 92 MiB.  Larger calls return
  * TSDF_ERR_CAPACITY with nothing enqueued.  Each GPU holds one such array
  * for the life of the process, shared by every engine on it: calls on one
  * GPU must not overlap (on one engine they are ordered by its stream; calls

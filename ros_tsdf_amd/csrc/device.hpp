@@ -57,9 +57,7 @@ struct EngCtr {
   int32_t new_n;      // entries inserted in this pass, waiting for heap blocks
   int32_t fold_next;  // large-block fold items handed out beyond the first per workgroup
   int32_t pad;
-  int32_t fold_n[8];  // touched blocks per fold size class, as their descriptors are filed
-  int32_t fold_tot[8];  // the same, counted before any descriptor is filed: class c's descriptors
-                        // start after those of the classes below it
+  int32_t fold_n[8];  // touched blocks per fold size class
   unsigned long long stats[ST_COUNT];
 };
 
@@ -106,8 +104,10 @@ struct Dev {
   uint32_t* fail_list;       // [entries] inserted entries without a heap block (removed after binning)
   uint32_t* new_list;        // [entries] entries inserted in this pass (heap blocks assigned after binning)
   uint8_t* new_map;          // [entries] their map slots
-  int4* fold_desc;           // [touched blocks] {record base, records, heap block, map}, in size-class order
+  int4* fold_desc;           // per size class c, from fold_base[c]: {record base, records, heap block, map}
   unsigned long long* part;  // [ST_COUNT][kPartSlots] statistics partial sums
+  int64_t fold_cap;
+  int64_t fold_base[8];      // first descriptor of each size class
   unsigned long long* seg;   // [records] update records, contiguous per touched block
   int64_t seg_cap;
   uint32_t* sel;             // [entries]

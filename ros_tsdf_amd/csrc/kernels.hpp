@@ -46,19 +46,17 @@ enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WGL, FC_WG, FC_WG2, FC_HUGE, FC_COU
 constexpr int kWgSplit = 128;  // FC_WGL: 65..128 records (a wave each), FC_WG: 129..kCountTileS
 static_assert(FC_COUNT <= (int)(sizeof(EngCtr::fold_n) / sizeof(int32_t)), "fold class counters");
 
-__host__ __device__ constexpr int fold_class(int n) {
-  return n <= 16 ? FC_16 : n <= 32 ? FC_32 : n <= 64 ? FC_64 : n <= kWgSplit ? FC_WGL : n <= kCountTileS ? FC_WG : n <= kCountTile ? FC_WG2 : FC_HUGE;
+// Descriptors a class can need when a pass holds at most `blocks` touched
+// blocks and `records` records: every block of class c holds at least lo
+// records (the class's lower size bound).
+__host__ __device__ constexpr int64_t fold_class_cap(int c, int64_t blocks, int64_t records) {
+  const int64_t lo = c == FC_16 ? 1 : c == FC_32 ? 17 : c == FC_64 ? 33 : c == FC_WGL ? 65 : c == FC_WG ? kWgSplit + 1 : c == FC_WG2 ? kCountTileS + 1 : kCountTile + 1;
+  const int64_t r = records / lo + 1;
+  return r < blocks ? r : blocks;
 }
 
-// First fold descriptor of size class c: the classes' descriptors follow
-// each other in class order, each class taking as many slots as k_pass_scan
-// counted touched blocks of it (an upper bound: blocks that get no heap
-// block are counted there and filed nowhere).  So the descriptors of a pass
-// need one slot per touched block.
-__device__ __forceinline__ int fold_class_base(const EngCtr* ctr, int c) {
-  int b = 0;
-  for (int k = 0; k < c; ++k) b += __hip_atomic_load(&ctr->fold_tot[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return b;
+__host__ __device__ constexpr int fold_class(int n) {
+  return n <= 16 ? FC_16 : n <= 32 ? FC_32 : n <= 64 ? FC_64 : n <= kWgSplit ? FC_WGL : n <= kCountTileS ? FC_WG : n <= kCountTile ? FC_WG2 : FC_HUGE;
 }
 
 // ---- wave / workgroup helpers -------------------------------------------------
@@ -243,10 +241,7 @@ __global__ void __launch_bounds__(256) k_pass_begin(Dev D, uint32_t mask, PassIn
       D.ctr->new_n = 0;
       D.ctr->fold_next = 0;
     }
-    if (t < FC_COUNT) {
-      D.ctr->fold_n[t] = 0;
-      D.ctr->fold_tot[t] = 0;
-    }
+    if (t < FC_COUNT) D.ctr->fold_n[t] = 0;
     if (inl.nscans >= 0) {
       if (t == 0) *const_cast<PassHdr*>(D.hdr) = inl.hdr;
       if (t < inl.nscans) const_cast<ScanDev*>(D.scans)[t] = inl.scans[t];
@@ -854,41 +849,19 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int32_t* n_ptr, Count 
 // come off the free stack with one pop, and one per-map count, per wave.
 __global__ void __launch_bounds__(256) k_pass_scan(Dev D) {
   __shared__ uint32_t sh[kScanSh];
-  __shared__ int lcls[FC_COUNT];
   const int n = ld_acq(&D.ctr->touched_n);
   const int ntiles = (n + kScanTile - 1) / kScanTile;
   const CountTouchedRecords cnt{D.touched, D.blk_cnt};
-  // touched blocks per fold size class: counted per thread, summed per wave,
-  // then per workgroup in LDS, and added to the pass totals once per workgroup
-  int ccls[FC_COUNT];
-#pragma unroll
-  for (int c = 0; c < FC_COUNT; ++c) ccls[c] = 0;
-  if (threadIdx.x < FC_COUNT) lcls[threadIdx.x] = 0;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     uint32_t v = 0;
 #pragma unroll
     for (int k = 0; k < kScanTile / 256; ++k) {
       const int i = t * kScanTile + k * 256 + threadIdx.x;
-      if (i < n) {
-        const uint32_t c = cnt(i);
-        v += c;
-        const int fc = fold_class((int)c);
-#pragma unroll
-        for (int q = 0; q < FC_COUNT; ++q) ccls[q] += fc == q ? 1 : 0;
-      }
+      if (i < n) v += cnt(i);
     }
     uint32_t tot;
-    block_scan_excl(v, sh, tot);  // (its barriers also order lcls's clearing before the adds below)
+    block_scan_excl(v, sh, tot);
     if (threadIdx.x == 0) D.tile_sums[t] = tot;
-  }
-  if ((int)blockIdx.x < ntiles) {
-#pragma unroll
-    for (int q = 0; q < FC_COUNT; ++q) {
-      const unsigned long long s = wave_sum((unsigned long long)ccls[q]);
-      if (lane_id() == 0 && s) atomicAdd(&lcls[q], (int)s);
-    }
-    __syncthreads();
-    if (threadIdx.x < FC_COUNT && lcls[threadIdx.x]) atomicAdd(&D.ctr->fold_tot[threadIdx.x], lcls[threadIdx.x]);
   }
   // heap failures include quota failures, which are also counted apart
   uint32_t st[3] = {0, 0, 0};
@@ -1048,8 +1021,7 @@ __device__ __forceinline__ int rec_point(unsigned long long r) { return (int)((r
 // the next pass, and entries that got no heap block leave the table.
 __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
   __shared__ unsigned long long sh64[kScanSh];
-  __shared__ int lcnt[FC_COUNT], gbase[FC_COUNT], cbase[FC_COUNT];
-  if (threadIdx.x < FC_COUNT) cbase[threadIdx.x] = fold_class_base(D.ctr, threadIdx.x);
+  __shared__ int lcnt[FC_COUNT], gbase[FC_COUNT];
   const int n = ld_acq(&D.ctr->touched_n);
   const int ntiles = (n + kScanTile - 1) / kScanTile;
   unsigned long long dropped = 0;
@@ -1110,7 +1082,7 @@ __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      if (cls[k] >= 0) D.fold_desc[cbase[cls[k]] + gbase[cls[k]] + rank[k]] = dsc[k];
+      if (cls[k] >= 0) D.fold_desc[D.fold_base[cls[k]] + gbase[cls[k]] + rank[k]] = dsc[k];
     __syncthreads();
   }
   const int nf = ld_acq(&D.ctr->fail_n);
@@ -1132,7 +1104,7 @@ __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int 
   const int gl = lane & (G - 1);
   const int grp = lane / G;
   const int n_desc = q_end;
-  const int4* desc = D.fold_desc + fold_class_base(D.ctr, kCls);
+  const int4* desc = D.fold_desc + D.fold_base[kCls];
   for (int q0 = q_begin; q0 < n_desc; q0 += kPer) {
     const int q = q0 + grp;
     int4 d = make_int4(0, 0, 0, 0);
@@ -1664,8 +1636,8 @@ __global__ void __launch_bounds__(256, 7) k_fold_a(Dev D) {
   const int cl = (n_wgl + kl - 1) / kl;
   const int c64 = (n64 + k64 - 1) / k64, c32 = (n32 + k32 - 1) / k32, c16 = (n16 + k16 - 1) / k16;
   const int total = n_wg + cl + c64 + c32 + c16;
-  const int4* dwg = D.fold_desc + fold_class_base(D.ctr, FC_WG);
-  const int4* dwgl = D.fold_desc + fold_class_base(D.ctr, FC_WGL);
+  const int4* dwg = D.fold_desc + D.fold_base[FC_WG];
+  const int4* dwgl = D.fold_desc + D.fold_base[FC_WGL];
   static_assert(kCountTileS >= 4 * kWgSplit, "a wave's strip of kWgSplit records in rin");
   for (int it = blockIdx.x; it < total; it += gridDim.x) {
     if (it < n_wg) {
@@ -1721,8 +1693,8 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
   unsigned long long uniq = 0;
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG2]);
   const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
-  const int4* desc = D.fold_desc + fold_class_base(D.ctr, FC_WG2);
-  const int4* hdesc = D.fold_desc + fold_class_base(D.ctr, FC_HUGE);
+  const int4* desc = D.fold_desc + D.fold_base[FC_WG2];
+  const int4* hdesc = D.fold_desc + D.fold_base[FC_HUGE];
   int q0 = 0;  // the first item taken from the counter
   if (n_huge == 0) {
     if ((int)blockIdx.x < n_wg)

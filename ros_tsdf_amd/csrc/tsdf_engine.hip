@@ -189,7 +189,8 @@ struct tsdf_engine {
   uint8_t* new_map = nullptr;
   int4* fold_desc = nullptr;
   unsigned long long* part = nullptr;
-  int64_t fold_cap = 0;  // fold descriptors: one per touched block of a pass
+  int64_t fold_cap = 0;
+  int64_t fold_base[8] = {};
   uint8_t* touched_map = nullptr;
   uint32_t* sel = nullptr;
   uint32_t* pub_cnt = nullptr;
@@ -254,6 +255,8 @@ struct tsdf_engine {
     D.new_map = new_map;
     D.fold_desc = fold_desc;
     D.part = part;
+    D.fold_cap = fold_cap;
+    for (int c = 0; c < 8; ++c) D.fold_base[c] = fold_base[c];
     D.touched_map = touched_map;
     D.seg = seg;
     D.seg_cap = seg_cap;
@@ -365,9 +368,14 @@ int ensure_scratch(tsdf_engine* E, int64_t P, int stride) {
   CHK(dev_alloc(&E->seg, np * ns));
   (void)hipFree(E->fold_desc);
   E->fold_desc = nullptr;
-  // a pass touches at most one block per record slot, and at most every entry
+  // one region per size class, each as large as that class can get
   E->fold_cap = std::min<int64_t>(E->entries, np * ns);
-  CHK(dev_alloc(&E->fold_desc, E->fold_cap));
+  int64_t fold_total = 0;
+  for (int c = 0; c < FC_COUNT; ++c) {
+    E->fold_base[c] = fold_total;
+    fold_total += fold_class_cap(c, E->fold_cap, np * ns);
+  }
+  CHK(dev_alloc(&E->fold_desc, fold_total));
   E->pcap = np;
   E->stride_cap = ns;
   E->seg_cap = np * ns;
@@ -1046,18 +1054,24 @@ int tsdf_integrate_noalloc(tsdf_map* m, const float* points_dev, int64_t n, cons
     if (!std::isfinite(sensor[k])) return fail(TSDF_ERR_INVALID_ARG, "non-finite sensor position");
   tsdf_engine* E = m->eng;
   // the pass scratch layout, as ensure_scratch lays it out for the engine's
-  // own scratch: pair rows and records with max_vox slots per point, one fold
-  // descriptor per block the pass can touch (at most the map's entries)
+  // own scratch: pair rows and records with max_vox slots per point, and one
+  // region of fold descriptors per size class, each as large as that class
+  // can get in a pass over this map (no more blocks than the map's entries)
   const int64_t pc = std::max<int64_t>(n, 1);
   const int64_t stride = m->desc.max_vox;
   const int64_t slots = pc * stride;
   const int64_t fold_cap = std::min<int64_t>(((int64_t)m->desc.n_home + m->desc.n_ovf) * kBucket, slots);
+  int64_t fold_base[FC_COUNT], fold_total = 0;
+  for (int c = 0; c < FC_COUNT; ++c) {
+    fold_base[c] = fold_total;
+    fold_total += fold_class_cap(c, fold_cap, slots);
+  }
   auto up = [](int64_t b) { return (b + 255) & ~int64_t(255); };
   const int64_t o_pe = 0;
   const int64_t o_pr = up(o_pe + (int64_t)sizeof(uint32_t) * slots);
   const int64_t o_seg = up(o_pr + (int64_t)sizeof(uint32_t) * slots);
   const int64_t o_fd = up(o_seg + (int64_t)sizeof(unsigned long long) * slots);
-  const int64_t o_pn = up(o_fd + (int64_t)sizeof(int4) * fold_cap);
+  const int64_t o_pn = up(o_fd + (int64_t)sizeof(int4) * fold_total);
   const int64_t bytes = up(o_pn + pc);
   if (bytes > kNoAllocScratchBytes)
     return fail(TSDF_ERR_CAPACITY,
@@ -1073,6 +1087,8 @@ int tsdf_integrate_noalloc(tsdf_map* m, const float* points_dev, int64_t n, cons
   D.pair_r = reinterpret_cast<uint32_t*>(base + o_pr);
   D.seg = reinterpret_cast<unsigned long long*>(base + o_seg);
   D.fold_desc = reinterpret_cast<int4*>(base + o_fd);
+  D.fold_cap = fold_cap;
+  for (int c = 0; c < FC_COUNT; ++c) D.fold_base[c] = fold_base[c];
   D.pair_n = base + o_pn;
   D.pcap = pc;
   D.seg_cap = slots;
