@@ -93,6 +93,15 @@ __device__ __forceinline__ void truncation_segment(const float p[3], const float
   }
 }
 
+// floor(x) as an integer, in one instruction (v_cvt_flr_i32_f32): the same
+// value as (int)floorf(x) for every x the point validity rule allows
+// (|x| < 2^31; NaN gives 0 both ways).
+__device__ __forceinline__ int32_t floor_int(float x) {
+  int32_t r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 // traverseVolume (H:146-245) for the cell keys: steps a point from a towards
 // b in increments of +-L, one or more axes at a time by the reference's tMax
 // comparison tree, and follows the integer cell k = floor(x * s), s = 1/L.
@@ -102,66 +111,178 @@ __device__ __forceinline__ void truncation_segment(const float p[3], const float
 // tests on k alone, because centres of adjacent cells are L apart up to FP32
 // rounding that the point validity rule keeps far below L/4.  Only the setup
 // (tMax from the centre of a's cell, tDelta) needs FP32 divisions.
-// At most max_cells cells are emitted; trunc reports the cut.
-// emit(k0, k1, k2) receives each cell and returns false to stop the walk
-// early (the caller has seen every cell it needs).
+struct CellWalk {
+  float step[3], tMax[3], tDelta[3], cur[3];
+  int32_t k[3], ke[3];
+  float scale;
+
+  __device__ __forceinline__ CellWalk(const float a[3], const float b[3], float L, float sc, float isc, float half)
+      : scale(sc) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float v = b[i] - a[i];
+      step[i] = v > 0 ? L : -1.0f * L;
+      const float f = floorf(a[i] * sc);
+      const float c0 = div_by_scale(f, sc, isc) + half;
+      tMax[i] = fabsf(v < 0 ? (c0 - half - a[i]) / v : (c0 + half - a[i]) / v);
+      tDelta[i] = fabsf(L / v);
+      cur[i] = a[i];
+      k[i] = (int32_t)f;
+      ke[i] = floor_int(b[i] * sc);
+    }
+  }
+
+  __device__ __forceinline__ bool at_end() const { return k[0] == ke[0] && k[1] == ke[1] && k[2] == ke[2]; }
+
+  // Moves to the next cell by the reference's comparison tree (H:175-227):
+  //   t0 < t1:  x if t0 < t2, z if t0 > t2, else x and z
+  //   t0 > t1:  y if t1 < t2, z if t1 > t2, else y and z
+  //   else:     z if t2 < t0, x and y if t2 > t0, else all three
+  // ("else" also covers comparisons with NaN, which are all false).  Written
+  // with its six distinct comparisons: x steps unless t0 > t1 or t0 > t2; z
+  // steps unless (t0 > t1 ? t1 : t0) < t2; y steps when t0 > t1 and not
+  // t1 > t2, or when t0, t1 are tied (neither < nor >) and not t0 > t2.
+  // Returns false, with the walk unchanged, when the precision guard ends
+  // it; otherwise xing[i] tells whether axis i entered a new block of 8 cells
+  // (with |kn - k| <= 1, the block index k >> 3 changes exactly when k ^ kn
+  // has a bit above the low three).
+  __device__ __forceinline__ bool advance(bool xing[3]) {
+    const bool gt01 = tMax[0] > tMax[1], lt01 = tMax[0] < tMax[1];
+    const bool gt02 = tMax[0] > tMax[2];
+    const bool gt12 = tMax[1] > tMax[2];
+    const bool sx = !gt01 && !gt02;
+    const bool sy = (gt01 && !gt12) || (!lt01 && !gt01 && !gt02);
+    const bool sz = !((gt01 ? tMax[1] : tMax[0]) < tMax[2]);
+    const bool st[3] = {sx, sy, sz};
+    int32_t kn[3], d[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      cur[i] = st[i] ? cur[i] + step[i] : cur[i];
+      tMax[i] = st[i] ? tMax[i] + tDelta[i] : tMax[i];
+      kn[i] = floor_int(cur[i] * scale);
+      d[i] = kn[i] - k[i];
+    }
+    // the guard: no axis moved a cell, or one moved more than one
+    const uint32_t far = max(max((uint32_t)(d[0] + 1), (uint32_t)(d[1] + 1)), (uint32_t)(d[2] + 1));
+    if ((d[0] | d[1] | d[2]) == 0 || far > 2u) return false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      xing[i] = (uint32_t)(kn[i] ^ k[i]) > 7u;
+      k[i] = kn[i];
+    }
+    return true;
+  }
+};
+
+// The cells of traverseVolume, in order: at most max_cells are emitted and
+// trunc reports the cut.  emit(k0, k1, k2) receives each cell and returns
+// false to stop the walk early (the caller has seen every cell it needs).
 template <class Emit>
 __device__ __forceinline__ int walk_cells(const float a[3], const float b[3], float L, float scale, float iscale,
                                           float half, int max_cells, bool& trunc, Emit&& emit) {
   trunc = false;
-  const float v[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-  float step[3], tMax[3], tDelta[3], cur[3];
-  int32_t k[3], ke[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    step[i] = v[i] > 0 ? L : -1.0f * L;
-    const float f = floorf(a[i] * scale);
-    const float c0 = div_by_scale(f, scale, iscale) + half;
-    tMax[i] = fabsf(v[i] < 0 ? (c0 - half - a[i]) / v[i] : (c0 + half - a[i]) / v[i]);
-    tDelta[i] = fabsf(L / v[i]);
-    cur[i] = a[i];
-    k[i] = (int32_t)f;
-    ke[i] = (int32_t)floorf(b[i] * scale);
-  }
+  CellWalk w(a, b, L, scale, iscale, half);
   int n = 0;
-  while (!(k[0] == ke[0] && k[1] == ke[1] && k[2] == ke[2])) {
+  while (!w.at_end()) {
     if (n == max_cells) {
       trunc = true;
       return n;
     }
-    const bool go = emit(k[0], k[1], k[2]);
+    const bool go = emit(w.k[0], w.k[1], w.k[2]);
     ++n;
     if (!go) return n;
-    // The reference's comparison tree (H:175-227), written without branches:
-    //   t0 < t1:  x if t0 < t2, z if t0 > t2, else x and z
-    //   t0 > t1:  y if t1 < t2, z if t1 > t2, else y and z
-    //   else:     z if t2 < t0, x and y if t2 > t0, else all three
-    // ("else" also covers comparisons with NaN, which are all false).
-    const bool A = tMax[0] < tMax[1], B = tMax[0] > tMax[1], E = !A && !B;
-    const bool sx = (A && !(tMax[0] > tMax[2])) || (E && !(tMax[2] < tMax[0]));
-    const bool sy = (B && !(tMax[1] > tMax[2])) || (E && !(tMax[2] < tMax[0]));
-    const bool sz = (A && !(tMax[0] < tMax[2])) || (B && !(tMax[1] < tMax[2])) || (E && !(tMax[2] > tMax[0]));
-    cur[0] = sx ? cur[0] + step[0] : cur[0];
-    tMax[0] = sx ? tMax[0] + tDelta[0] : tMax[0];
-    cur[1] = sy ? cur[1] + step[1] : cur[1];
-    tMax[1] = sy ? tMax[1] + tDelta[1] : tMax[1];
-    cur[2] = sz ? cur[2] + step[2] : cur[2];
-    tMax[2] = sz ? tMax[2] + tDelta[2] : tMax[2];
-    int32_t kn[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) kn[i] = (int32_t)floorf(cur[i] * scale);
-    const int32_t d0 = abs(kn[0] - k[0]), d1 = abs(kn[1] - k[1]), d2 = abs(kn[2] - k[2]);
-    if ((d0 == 0 && d1 == 0 && d2 == 0) || d0 > 1 || d1 > 1 || d2 > 1) return n;
-    k[0] = kn[0];
-    k[1] = kn[1];
-    k[2] = kn[2];
+    bool x[3];
+    if (!w.advance(x)) return n;
   }
   if (n == max_cells) {
     trunc = true;
     return n;
   }
-  emit(k[0], k[1], k[2]);
+  emit(w.k[0], w.k[1], w.k[2]);
   return n + 1;
+}
+
+// The same walk (voxel cells, blocks of 8^3) summarised as runs: every axis
+// moves monotonically, so the cells of one block are consecutive and the
+// walk is a sequence of runs, one per block it passes through.
+// on_run(kb0, kb1, kb2, len) is called for each run in walk order.  Per cell
+// the walk only notes, in three bit masks over a window of 32 cells, which
+// axes entered a new block; the runs are read off the masks when the window
+// fills and at the end.  Returns the number of cells (as walk_cells).
+template <class OnRun>
+__device__ __forceinline__ int walk_runs(const float a[3], const float b[3], float L, float scale, float iscale,
+                                         float half, int max_cells, bool& trunc, OnRun&& on_run) {
+  trunc = false;
+  CellWalk w(a, b, L, scale, iscale, half);
+  int32_t blk[3] = {w.k[0] >> 3, w.k[1] >> 3, w.k[2] >> 3};  // block of the open run
+  const int32_t dir[3] = {w.step[0] > 0 ? 1 : -1, w.step[1] > 0 ? 1 : -1, w.step[2] > 0 ? 1 : -1};
+  int open = 0;  // cells of the open run before the window
+  int n = 0;     // cells so far
+  // the runs ending among window cells [base, base + len): cm[i] bit j tells
+  // that window cell j entered a new block along axis i
+  auto drain = [&](const uint32_t cm[3], int len) {
+    uint32_t m = cm[0] | cm[1] | cm[2];
+    if (len < 32) m &= (1u << len) - 1u;
+    int prev = 0;
+    while (m) {
+      const int p = __builtin_ctz(m);
+      if (open + p - prev > 0) on_run(blk[0], blk[1], blk[2], open + p - prev);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) blk[i] += (cm[i] >> p) & 1u ? dir[i] : 0;
+      open = 0;
+      prev = p;
+      m &= m - 1u;
+    }
+    open += len - prev;
+  };
+  if (max_cells <= 32) {
+    // every cell fits one window: a loop with no exit but the walk's own end
+    uint32_t cm[3] = {0u, 0u, 0u};
+    for (;;) {
+      if (n == max_cells) {  // (the end cell too: walk_cells cuts it as well)
+        trunc = true;
+        break;
+      }
+      const bool last = w.at_end();
+      ++n;
+      bool x[3];
+      if (last || !w.advance(x)) break;
+      // now at cell n; a cell past max_cells (n = 32) is never counted
+      const uint32_t bit = n < 32 ? 1u << n : 0u;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) cm[i] = x[i] ? cm[i] | bit : cm[i];
+    }
+    drain(cm, n);
+    if (open > 0) on_run(blk[0], blk[1], blk[2], open);
+    return n;
+  }
+  bool more = true;
+  bool x[3] = {false, false, false};  // block entries of the cell that opens the window
+  for (int base = 0; more; base = n) {
+    // bit j of cm[i]: window cell j entered a new block along axis i
+    uint32_t cm[3] = {x[0] ? 1u : 0u, x[1] ? 1u : 0u, x[2] ? 1u : 0u};
+    // each turn: the current cell n is counted, then the walk moves on
+    for (;;) {
+      if (n == max_cells) {  // (the end cell too: walk_cells cuts it as well)
+        trunc = true;
+        more = false;
+        break;
+      }
+      const bool last = w.at_end();
+      ++n;
+      if (last || !w.advance(x)) {
+        more = false;
+        break;
+      }
+      if (n - base == 32) break;  // cell n opens the next window
+      const uint32_t bit = 1u << (n - base);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) cm[i] = x[i] ? cm[i] | bit : cm[i];
+    }
+    drain(cm, n - base);
+  }
+  if (open > 0) on_run(blk[0], blk[1], blk[2], open);
+  return n;
 }
 
 // Centre of cell k: getVolumeCenterFromPoint (H:76-85) with floor(x * s)

@@ -342,17 +342,6 @@ __device__ __forceinline__ unsigned long long lds_block_key(int map_slot, const 
          ((unsigned long long)(kb[1] & 0xFFFF) << 16) | (unsigned long long)(kb[2] & 0xFFFF);
 }
 
-// Run records of the voxel walk (see k_bin): a run is at most 22 voxels (the
-// longest monotone path through an 8^3 block), so 6 bits of length suffice.
-constexpr int kPackedRuns = 10;
-__device__ __forceinline__ uint32_t run_code(int d0, int d1, int d2) {
-  return (uint32_t)(d0 & 3) | ((uint32_t)(d1 & 3) << 2) | ((uint32_t)(d2 & 3) << 4);
-}
-__device__ __forceinline__ int run_step(uint32_t code, int axis) {
-  const int v = (int)(code >> (2 * axis)) & 3;
-  return v == 3 ? -1 : v;
-}
-
 // Slot of an LDS-table key: the 48 coordinate bits and the map folded into
 // 32 bits and mixed with one multiply (the table only spreads the few
 // hundred blocks of one workgroup, so a light hash suffices).
@@ -605,59 +594,14 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D) {
         }
         ++nj;
       };
-      // The voxel walk only records its runs, in two registers: 6 bits of
-      // length per run, and 6 bits of block step from the previous run (2
-      // bits per axis: consecutive runs' blocks differ by at most one per
-      // axis).  The table work for the runs follows the walk, so the lanes
-      // of a wave do it together rather than at scattered steps of their
-      // walks.  A walk of more than kPackedRuns runs flushes mid-way.
-      unsigned long long lens = 0, steps = 0;
-      int nrun = 0, rn = 0;
-      int32_t fb[3] = {0, 0, 0}, pb[3] = {0, 0, 0};  // flush base block, current run's block
-      auto flush_runs = [&]() {
-        int32_t cb[3] = {fb[0], fb[1], fb[2]};
-        for (int j = 0; j < nrun; ++j) {
-          const uint32_t code = (uint32_t)(steps >> (6 * j)) & 63u;
-          cb[0] += run_step(code, 0);
-          cb[1] += run_step(code, 1);
-          cb[2] += run_step(code, 2);
-          close_run(cb, (int)((lens >> (6 * j)) & 63u));
-        }
-        nrun = 0;
-        lens = 0;
-        steps = 0;
-        fb[0] = cb[0];
-        fb[1] = cb[1];
-        fb[2] = cb[2];
-      };
+      // The voxel walk notes only which cells enter a new block (walk_runs);
+      // the table work for its runs follows the walk, so the lanes of a wave
+      // do it together rather than at scattered steps of their walks.
       bool tv = false;
-      walk_cells(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
-        const int32_t kb[3] = {block_of(k0), block_of(k1), block_of(k2)};
-        if (rn == 0) {
-          fb[0] = pb[0] = kb[0];
-          fb[1] = pb[1] = kb[1];
-          fb[2] = pb[2] = kb[2];
-          nrun = 1;
-          rn = 1;
-          return true;
-        }
-        if (kb[0] != pb[0] || kb[1] != pb[1] || kb[2] != pb[2]) {
-          lens |= (unsigned long long)rn << (6 * (nrun - 1));
-          if (nrun == kPackedRuns) flush_runs();
-          steps |= (unsigned long long)run_code(kb[0] - pb[0], kb[1] - pb[1], kb[2] - pb[2]) << (6 * nrun);
-          ++nrun;
-          rn = 0;
-          pb[0] = kb[0];
-          pb[1] = kb[1];
-          pb[2] = kb[2];
-        }
-        ++rn;
-        return true;
+      walk_runs(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t b0, int32_t b1, int32_t b2, int rn) {
+        const int32_t kb[3] = {b0, b1, b2};
+        close_run(kb, rn);
       });
-      if (nrun > 0) {
-        lens |= (unsigned long long)rn << (6 * (nrun - 1));
-        flush_runs();
-      }
       st[4] += tv ? 1 : 0;
     }
     if (i < P) D.pair_n[i] = (uint8_t)nj;
@@ -937,38 +881,43 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
     const float s[3] = {sd.s[0], sd.s[1], sd.s[2]};
     float a[3], b[3];
     truncation_segment(p, s, m.T, a, b);
-    int32_t pb0 = INT32_MIN, pb1 = 0, pb2 = 0;  // block of the current run
-    int j = -1;
-    int32_t at = -1;
-    bool tv;
+    // the walk of k_bin's voxel DDA; a cell that enters a new block starts
+    // the next run (runs are in walk order in the pair rows).  rq holds the
+    // record offsets of the current run and the next ones; a run's record of
+    // cell n goes to rq[0] + n - (the run's first cell).
+    int32_t rq[kEmitRuns];
+#pragma unroll
+    for (int r = 0; r < kEmitRuns; ++r) rq[r] = roff[r];
+    int j = 0;   // current run
+    int rs = 0;  // its first cell
     // kProduct: every lane's map forms k / scale by the checked product (a
     // product and two FMAs per axis, no branch); otherwise each axis picks
     // its map's form.  The choice is made once per wave.
     auto walk = [&](auto product) {
       constexpr bool kProduct = decltype(product)::value;
-      walk_cells(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t k0, int32_t k1, int32_t k2) {
-        const int32_t b0 = block_of(k0), b1 = block_of(k1), b2 = block_of(k2);
-        if (b0 != pb0 || b1 != pb1 || b2 != pb2) {
-          pb0 = b0;
-          pb1 = b1;
-          pb2 = b2;
-          ++j;
-          at = j < kEmitRuns ? -1 : run_offset(D, j, i);
-#pragma unroll
-          for (int r = 0; r < kEmitRuns; ++r)
-            if (r == j) at = roff[r];
-        }
-        if (at >= 0) {
-          const int32_t kk[3] = {k0, k1, k2};
+      CellWalk w(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs);
+      // each turn: the current cell is recorded, then the walk moves on
+      // (walk_cells' cells: at most max_vox, the end cell included)
+      for (int n = 0; n < m.max_vox; ++n) {
+        const bool last = w.at_end();
+        if (rq[0] >= 0) {
           float vc[3];
 #pragma unroll
           for (int u = 0; u < 3; ++u)
-            vc[u] = kProduct ? product_centre(kk[u], m.sc_v, m.isc_v, m.half_vs)
-                             : cell_centre_of(kk[u], m.sc_v, m.isc_v, m.half_vs);
-          D.seg[(uint32_t)at++] = make_rec(local_index(kk), i, signed_distance(p, s, vc));
+            vc[u] = kProduct ? product_centre(w.k[u], m.sc_v, m.isc_v, m.half_vs)
+                             : cell_centre_of(w.k[u], m.sc_v, m.isc_v, m.half_vs);
+          D.seg[(uint32_t)(rq[0] + n - rs)] = make_rec(local_index(w.k), i, signed_distance(p, s, vc));
         }
-        return true;
-      });
+        bool x[3];
+        if (last || !w.advance(x)) break;
+        if (x[0] || x[1] || x[2]) {
+          ++j;
+          rs = n + 1;
+#pragma unroll
+          for (int r = 0; r + 1 < kEmitRuns; ++r) rq[r] = rq[r + 1];
+          rq[kEmitRuns - 1] = j + kEmitRuns - 1 < nj ? run_offset(D, j + kEmitRuns - 1, i) : -1;
+        }
+      }
     };
     if (__ballot(m.isc_v == 0.0f) == 0) walk(std::true_type{});
     else walk(std::false_type{});
