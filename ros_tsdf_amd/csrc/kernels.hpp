@@ -823,6 +823,17 @@ __global__ void __launch_bounds__(256) k_pass_scan(Dev D) {
   flush_stats(D, which, st);
 }
 
+// Lanes of one wave exchange data through LDS: the wave's LDS requests
+// complete in order, so waiting for the wave's own LDS traffic and keeping
+// the compiler from moving memory accesses across the point is enough; no
+// global-memory fence (and no wait on outstanding stores) is involved.
+__device__ __forceinline__ void wave_sync() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // ---- records ---------------------------------------------------------------------------
 // Record of one voxel update: local voxel (9 bits) | point index in the pass
 // (23 bits) | FP32 signed distance.  Sorting records as integers orders them
@@ -843,14 +854,35 @@ __device__ __forceinline__ int32_t run_offset(const Dev& D, int r, int i) {
 // (getDistanceUpdate, H:562-573) is written to its block's record segment at
 // the offset its run reserved in k_bin, so every block's records end up
 // contiguous.
+// A wave's lanes walk different rays, so the records of one walk step go to
+// 64 unrelated places, and a store of them touches 64 cache lines.  The
+// records are therefore staged in LDS, kEmitChunk walk steps at a time,
+// lane-major: slot lane * kEmitChunk + step.  Consecutive slots are then
+// consecutive records of one run (contiguous in the segment), and the wave
+// writes the staged slots out 64 at a time, each store covering the runs of
+// a few lanes.  Large passes, whose stores bound the kernel, stage
+// (kStage); a single scan's pass, two waves per SIMD and bound by latency,
+// stores each record as it is made.
+constexpr int kEmitChunk = 8;
+constexpr int kEmitStagePoints = 1 << 18;  // passes of more points use the staged variant
+template <bool kStage>
 __global__ void __launch_bounds__(256) k_emit(Dev D) {
+  __shared__ unsigned long long srec[kStage ? 4 : 1][kStage ? 64 * kEmitChunk : 1];
+  __shared__ int32_t sdst[kStage ? 4 : 1][kStage ? 64 * kEmitChunk : 1];
+  const int lane = lane_id();
+  unsigned long long* wrec = srec[kStage ? threadIdx.x >> 6 : 0];
+  int32_t* wdst = sdst[kStage ? threadIdx.x >> 6 : 0];
   const PassHdr hdr = *D.hdr;
   const int P = hdr.total;
   const int64_t stride = D.pcap;
   // run rows that exist in the pair arrays (they hold hdr.stride rows)
   const int rows = min(kEmitRuns, hdr.stride);
-  const bool wide = P > (1 << 18);  // a pass of more than two 131k-point scans
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
+  const bool wide = P > kEmitStagePoints;  // a pass of more than two 131k-point scans
+  // whole waves walk the grid, so that the lanes of a wave stage and write
+  // out their records together
+  for (int wbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63); wbase < P; wbase += gridDim.x * blockDim.x) {
+    const int i = wbase + lane;
+    const bool have = i < P;
     // absolute record offsets of the point's runs, loaded before the walk
     // (the first kEmitRuns in registers; later runs, which need a walk far
     // longer than the truncation band of the workloads, are looked up when
@@ -858,65 +890,96 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
     // the blocks' record bases are one dependent load away -- in a large
     // (batched) pass only the first three rows: a point has 2-3 runs, and
     // the unused rows would be most of the kernel's reads.
-    const int nj = D.pair_n[i];
+    const int nj = have ? D.pair_n[i] : 0;
     uint32_t pe[kEmitRuns], pr[kEmitRuns];
 #pragma unroll
     for (int r = 0; r < kEmitRuns; ++r) {
       pe[r] = kPairFail;
       pr[r] = 0;
-      if (r < rows && (!wide || r < 3 || r < nj)) {
+      if (have && r < rows && (!wide || r < 3 || r < nj)) {
         pe[r] = D.pair_e[(int64_t)r * stride + i];
         pr[r] = D.pair_r[(int64_t)r * stride + i];
       }
     }
-    if (nj == 0) continue;
-    int32_t roff[kEmitRuns];
+    int32_t rq[kEmitRuns];  // record offsets of the current run and the next ones
 #pragma unroll
     for (int r = 0; r < kEmitRuns; ++r)
-      roff[r] = r < nj && pe[r] != kPairFail ? (int32_t)(D.blk_base[pe[r]] + pr[r]) : -1;
-    const ScanDev sd = D.scans[find_scan(D.scans, hdr.nscans, i)];
+      rq[r] = r < nj && pe[r] != kPairFail ? (int32_t)(D.blk_base[pe[r]] + pr[r]) : -1;
+    const int ic = have ? i : 0;
+    const ScanDev sd = D.scans[find_scan(D.scans, hdr.nscans, ic)];
     const MapDev m = D.maps[sd.map];
-    const float4 q = hdr.points[i];
+    const float4 q = hdr.points[ic];
     const float p[3] = {q.x, q.y, q.z};
     const float s[3] = {sd.s[0], sd.s[1], sd.s[2]};
     float a[3], b[3];
     truncation_segment(p, s, m.T, a, b);
-    // the walk of k_bin's voxel DDA; a cell that enters a new block starts
-    // the next run (runs are in walk order in the pair rows).  rq holds the
-    // record offsets of the current run and the next ones; a run's record of
-    // cell n goes to rq[0] + n - (the run's first cell).
-    int32_t rq[kEmitRuns];
-#pragma unroll
-    for (int r = 0; r < kEmitRuns; ++r) rq[r] = roff[r];
+    // the walk of k_bin's voxel DDA (walk_cells' cells: at most max_vox,
+    // the end cell included); a cell that enters a new block starts the next
+    // run (runs are in walk order in the pair rows), and the run's record of
+    // cell n goes to rq[0] + n - (the run's first cell)
     int j = 0;   // current run
     int rs = 0;  // its first cell
+    int n = 0;   // cells so far
+    bool alive = nj > 0 && m.max_vox > 0;
     // kProduct: every lane's map forms k / scale by the checked product (a
     // product and two FMAs per axis, no branch); otherwise each axis picks
     // its map's form.  The choice is made once per wave.
     auto walk = [&](auto product) {
       constexpr bool kProduct = decltype(product)::value;
       CellWalk w(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs);
-      // each turn: the current cell is recorded, then the walk moves on
-      // (walk_cells' cells: at most max_vox, the end cell included)
-      for (int n = 0; n < m.max_vox; ++n) {
-        const bool last = w.at_end();
-        if (rq[0] >= 0) {
-          float vc[3];
+      auto record = [&](unsigned long long& rec) -> int32_t {
+        if (rq[0] < 0) return -1;
+        float vc[3];
 #pragma unroll
-          for (int u = 0; u < 3; ++u)
-            vc[u] = kProduct ? product_centre(w.k[u], m.sc_v, m.isc_v, m.half_vs)
-                             : cell_centre_of(w.k[u], m.sc_v, m.isc_v, m.half_vs);
-          D.seg[(uint32_t)(rq[0] + n - rs)] = make_rec(local_index(w.k), i, signed_distance(p, s, vc));
-        }
-        bool x[3];
-        if (last || !w.advance(x)) break;
-        if (x[0] || x[1] || x[2]) {
-          ++j;
-          rs = n + 1;
+        for (int u = 0; u < 3; ++u)
+          vc[u] = kProduct ? product_centre(w.k[u], m.sc_v, m.isc_v, m.half_vs)
+                           : cell_centre_of(w.k[u], m.sc_v, m.isc_v, m.half_vs);
+        rec = make_rec(local_index(w.k), i, signed_distance(p, s, vc));
+        return rq[0] + n - rs;
+      };
+      auto next_run = [&]() {
+        ++j;
+        rs = n + 1;
 #pragma unroll
-          for (int r = 0; r + 1 < kEmitRuns; ++r) rq[r] = rq[r + 1];
-          rq[kEmitRuns - 1] = j + kEmitRuns - 1 < nj ? run_offset(D, j + kEmitRuns - 1, i) : -1;
+        for (int r = 0; r + 1 < kEmitRuns; ++r) rq[r] = rq[r + 1];
+        rq[kEmitRuns - 1] = j + kEmitRuns - 1 < nj ? run_offset(D, j + kEmitRuns - 1, i) : -1;
+      };
+      if (!kStage) {
+        // each turn: the current cell's record is stored, then the walk moves on
+        for (; alive && n < m.max_vox; ++n) {
+          const bool last = w.at_end();
+          unsigned long long rec;
+          const int32_t dst = record(rec);
+          if (dst >= 0) D.seg[(uint32_t)dst] = rec;
+          bool x[3];
+          if (last || !w.advance(x)) break;
+          if (x[0] || x[1] || x[2]) next_run();
         }
+        return;
+      }
+      while (__ballot(alive)) {
+#pragma unroll
+        for (int st = 0; st < kEmitChunk; ++st) {
+          int32_t dst = -1;
+          unsigned long long rec = 0;
+          if (alive) {
+            const bool last = w.at_end();
+            dst = record(rec);
+            bool x[3];
+            if (last || !w.advance(x)) alive = false;
+            else if (x[0] || x[1] || x[2]) next_run();
+            if (++n >= m.max_vox) alive = false;
+          }
+          wrec[lane * kEmitChunk + st] = rec;
+          wdst[lane * kEmitChunk + st] = dst;
+        }
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < kEmitChunk; ++r) {
+          const int32_t dst = wdst[r * 64 + lane];
+          if (dst >= 0) D.seg[(uint32_t)dst] = wrec[r * 64 + lane];
+        }
+        wave_sync();
       }
     };
     if (__ballot(m.isc_v == 0.0f) == 0) walk(std::true_type{});
@@ -925,16 +988,6 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
 }
 
 // ---- per-block fold ---------------------------------------------------------------------
-// Lanes of one wave exchange data through LDS: the wave's LDS requests
-// complete in order, so waiting for the wave's own LDS traffic and keeping
-// the compiler from moving memory accesses across the point is enough; no
-// global-memory fence (and no wait on outstanding stores) is involved.
-__device__ __forceinline__ void wave_sync() {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-  __builtin_amdgcn_wave_barrier();
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
 
 __device__ __forceinline__ int next_pow2(int c) {
   int n = 2;

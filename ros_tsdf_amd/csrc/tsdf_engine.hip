@@ -468,7 +468,8 @@ int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int ti
   hipLaunchKernelGGL(k_pass_scan, dim3(sg), dim3(256), 0, st, D);
   hipLaunchKernelGGL(k_seg_apply, dim3(sg), dim3(256), 0, st, D);
   CHK(timing_mark(E, tid, 3));
-  hipLaunchKernelGGL(k_emit, dim3(grid_for(P, 256, 65536)), dim3(256), 0, st, D);
+  if (P > kEmitStagePoints) hipLaunchKernelGGL(k_emit<true>, dim3(grid_for(P, 256, 65536)), dim3(256), 0, st, D);
+  else hipLaunchKernelGGL(k_emit<false>, dim3(grid_for(P, 256, 65536)), dim3(256), 0, st, D);
   CHK(timing_mark(E, tid, 4));
   const int fg = std::max(1, L.fold_grid);
   with_state(E->state_format, [&](auto s) {
