@@ -237,9 +237,9 @@ int tsdf_batch_graph_destroy(tsdf_batch_graph* g);
 
 /* Launch-shape knobs of the batched pipeline, set by the autotuner. */
 typedef struct {
-  int32_t traverse_block;    /* threads per workgroup of the traversal kernel (64..1024) */
-  int32_t traverse_grid_cap; /* max workgroups of the traversal kernel */
-  int32_t fold_grid;         /* workgroups of the first fold launch (its work items are spread over them) */
+  int32_t traverse_block;    /* threads per workgroup of the traversal kernel: 64, 128, 192 or 256 */
+  int32_t traverse_grid_cap; /* max workgroups of the traversal kernel (1..65536) */
+  int32_t fold_grid;         /* workgroups of the first fold launch, 1..65536 (its work items are spread over them) */
   int32_t reserved;
 } tsdf_launch_config;
 int tsdf_engine_set_launch(tsdf_engine* eng, const tsdf_launch_config* cfg);

@@ -24,6 +24,20 @@ enum Stat {
   ST_QUOTA_FAIL, ST_COUNT
 };
 
+// Statistics of an integration pass: every workgroup of a pass kernel writes
+// its own row of counts (no atomics, and nothing to clear between passes),
+// and the rows are summed on demand.  Each kernel has a region of rows, one
+// per workgroup of its largest grid.
+enum StatRegion { SR_BIN = 0, SR_SCAN, SR_SEG, SR_FOLD_A, SR_FOLD_B, SR_COUNT };
+constexpr int kStatCols = 8;  // counts per row
+constexpr int kStatRows[SR_COUNT] = {65536, 256, 256, 65536, 512};
+__host__ __device__ constexpr int stat_region_base(int r) {
+  int b = 0;
+  for (int k = 0; k < r; ++k) b += kStatRows[k];
+  return b;
+}
+constexpr int kStatRowsTotal = stat_region_base(SR_COUNT);
+
 // Per-map constants, resident in device memory (one entry per map slot).
 struct MapDev {
   float vs, T, maxW, pubD2, gcD2;
@@ -105,7 +119,8 @@ struct Dev {
   uint32_t* new_list;        // [entries] entries inserted in this pass (heap blocks assigned after binning)
   uint8_t* new_map;          // [entries] their map slots
   int4* fold_desc;           // per size class c, from fold_base[c]: {record base, records, heap block, map}
-  unsigned long long* part;  // [ST_COUNT][kPartSlots] statistics partial sums
+  unsigned long long* part;  // [ST_COUNT][kPartSlots] statistics partial sums (publish and gc)
+  uint32_t* srows;           // [kStatRowsTotal][kStatCols] pass statistics, a row per workgroup
   int64_t fold_cap;
   int64_t fold_base[8];      // first descriptor of each size class
   unsigned long long* seg;   // [records] update records, contiguous per touched block

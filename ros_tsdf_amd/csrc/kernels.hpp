@@ -1,8 +1,6 @@
 // Kernels of the integration pipeline.  One pass integrates the scans of a
 // batch (a single scan on the per-scan path):
 //
-//   k_pass_begin     zero the pass counters and statistics partials; store
-//                    the descriptors of a pass of up to 32 scans
 //   k_bin            per point: truncation segment, block DDA and voxel DDA;
 //                    find-or-insert every block in the map's spatial hash
 //                    (lock-free) once per workgroup; each run of voxels in
@@ -23,8 +21,8 @@
 //   k_fold_b         blocks of 1025..4096 records (counting sort), then
 //                    larger ones in voxel ranges that fit the counting tile
 // Every voxel's state is read once and written once per pass, its updates
-// folded in point order.  Statistics are summed into spread partial slots
-// and reduced on demand (k_stats_reduce).
+// folded in point order.  Pass statistics are per-workgroup rows, summed on
+// demand (k_pass_stats).
 //
 // publish: k_select (radius test per entry) -> k_pub_count (wave per block)
 //          -> scan -> k_pub_write (wave ballot compaction)
@@ -93,10 +91,71 @@ __device__ __forceinline__ void stat_add(const Dev& D, int stat, unsigned long l
   }
 }
 
-template <class T, int N>
-__device__ __forceinline__ void flush_stats(const Dev& D, const int* which, const T (&v)[N]) {
+
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) stat_add(D, which[i], v[i]);
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// The workgroup's row of pass statistics in region `region`: each thread's
+// counts v[0..N) summed over the workgroup.  Every workgroup writes its row,
+// zeros included, so rows need no clearing.  All threads must call this.
+template <int N>
+__device__ __forceinline__ void write_stat_row(const Dev& D, int region, const uint32_t (&v)[N]) {
+  static_assert(N <= kStatCols, "row width");
+  __shared__ uint32_t wsum[16][kStatCols];  // up to 16 waves (1024 threads)
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const uint32_t x = wave_sum32(v[k]);
+    if (lane_id() == 0) wsum[w][k] = x;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < N) {
+    uint32_t t = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += wsum[k][threadIdx.x];
+    D.srows[(int64_t)(stat_region_base(region) + blockIdx.x) * kStatCols + threadIdx.x] = t;
+  }
+}
+
+// EngCtr::stats of the last pass: the rows of each kernel's workgroups
+// summed (grid sizes of that pass's launches, by region), one workgroup per
+// statistic in mask.
+struct StatGrids {
+  int32_t n[SR_COUNT];
+};
+__global__ void __launch_bounds__(256) k_pass_stats(Dev D, StatGrids g, uint32_t mask) {
+  __shared__ unsigned long long sh[4];
+  const int st = blockIdx.x;
+  if (st >= ST_COUNT || !(mask & (1u << st))) return;
+  // (region, column) pairs that hold the statistic
+  int reg[2] = {-1, -1}, col[2] = {0, 0};
+  switch (st) {
+    case ST_VALID: reg[0] = SR_BIN; col[0] = 0; break;
+    case ST_SKIPPED: reg[0] = SR_BIN; col[0] = 1; break;
+    case ST_BLOCK_CELLS: reg[0] = SR_BIN; col[0] = 2; break;
+    case ST_UPDATES: reg[0] = SR_BIN; col[0] = 3; break;
+    case ST_TRUNC_VOX: reg[0] = SR_BIN; col[0] = 4; break;
+    case ST_TRUNC_BLK: reg[0] = SR_BIN; col[0] = 5; break;
+    case ST_INSERT_FAIL: reg[0] = SR_BIN; col[0] = 6; break;
+    case ST_UPD_DROPPED: reg[0] = SR_BIN; col[0] = 7; reg[1] = SR_SEG; col[1] = 0; break;
+    case ST_NEW_BLOCKS: reg[0] = SR_SCAN; col[0] = 0; break;
+    case ST_HEAP_FAIL: reg[0] = SR_SCAN; col[0] = 1; break;
+    case ST_QUOTA_FAIL: reg[0] = SR_SCAN; col[0] = 2; break;
+    case ST_UNIQUE: reg[0] = SR_FOLD_A; col[0] = 0; reg[1] = SR_FOLD_B; col[1] = 0; break;
+    default: break;  // ST_INDEX_DROPPED: no kernel counts it (valid points never produce it)
+  }
+  unsigned long long v = 0;
+  for (int q = 0; q < 2; ++q) {
+    if (reg[q] < 0) continue;
+    const uint32_t* r = D.srows + (int64_t)stat_region_base(reg[q]) * kStatCols + col[q];
+    for (int x = threadIdx.x; x < g.n[reg[q]]; x += blockDim.x) v += r[(int64_t)x * kStatCols];
+  }
+  v = wave_sum(v);
+  if (lane_id() == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) D.ctr->stats[st] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
 // Zeroes the partial slots of the statistics selected by mask.
@@ -219,12 +278,17 @@ __device__ inline int find_or_insert(const Dev& D, const MapDev& m, unsigned lon
 }
 
 // ---- pass kernels -----------------------------------------------------------------
-// Zeroes the pass counters and the partial slots of the statistics in mask.
 // A pass of at most kInlineScans scans brings its header and scan
-// descriptors as kernel arguments, and k_pass_begin stores them for the
-// pass's other kernels: no host-to-device copy precedes the pass.  nscans < 0:
-// the descriptors were copied into place already (larger batches, and graph
-// replays, whose captured arguments would be stale).
+// descriptors as k_bin arguments; k_bin's first workgroup stores them for
+// the pass's later kernels, so no host-to-device copy and no separate launch
+// precede the pass (a short first launch left the GPU idle while the host
+// issued k_bin).  nscans < 0: the descriptors were copied into place already
+// (larger batches, and graph replays, whose captured arguments would be
+// stale).
+// The pass counters need no launch to clear them either: each is cleared
+// after its last reader, by one thread of a later kernel (touched_n, new_n
+// and fail_n by k_emit, the fold counters by the next pass's k_bin), and
+// pass statistics are per-workgroup rows that every pass overwrites.
 constexpr int kInlineScans = 32;
 struct PassInline {
   PassHdr hdr;
@@ -232,23 +296,23 @@ struct PassInline {
   ScanDev scans[kInlineScans];
 };
 
-__global__ void __launch_bounds__(256) k_pass_begin(Dev D, uint32_t mask, PassInline inl) {
-  if (blockIdx.x == 0) {
-    const int t = threadIdx.x;
-    if (t == 0) {
-      D.ctr->touched_n = 0;
-      D.ctr->fail_n = 0;
-      D.ctr->new_n = 0;
-      D.ctr->fold_next = 0;
+// The scan of pass point i, from the inline descriptors or the stored ones.
+__device__ __forceinline__ ScanDev scan_of(const Dev& D, const PassInline& inl, int ns, int i) {
+  int lo = 0, hi = ns - 1;
+  if (inl.nscans >= 0) {
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (inl.scans[mid].off <= i) lo = mid;
+      else hi = mid - 1;
     }
-    if (t < FC_COUNT) D.ctr->fold_n[t] = 0;
-    if (inl.nscans >= 0) {
-      if (t == 0) *const_cast<PassHdr*>(D.hdr) = inl.hdr;
-      if (t < inl.nscans) const_cast<ScanDev*>(D.scans)[t] = inl.scans[t];
-    }
+    return inl.scans[lo];
   }
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ST_COUNT * kPartSlots; i += gridDim.x * blockDim.x)
-    if (mask & (1u << (i / kPartSlots))) D.part[i] = 0;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (D.scans[mid].off <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return D.scans[lo];
 }
 
 __device__ __forceinline__ int find_scan(const ScanDev* scans, int ns, int i) {
@@ -406,7 +470,7 @@ __device__ __forceinline__ int peek_find(const HomePeek& h, unsigned long long k
 // 168 VGPRs at most: three waves per SIMD where the grid has them (batched
 // passes; a 131k-point scan launches two per SIMD whatever the budget), with
 // a handful of spills off the walks.
-__global__ void __launch_bounds__(256, 3) k_bin(Dev D) {
+__global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl) {
   __shared__ unsigned long long tkey[kBinSlots];
   __shared__ int tent[kBinSlots];
   __shared__ uint32_t trec[kBinSlots];
@@ -417,11 +481,23 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D) {
   __shared__ uint8_t ltouch_map[kBinSlots];
   __shared__ uint16_t locc[kBinSlots];
   __shared__ int n_new, n_touch, g_touch, g_new, n_occ;
-  const PassHdr hdr = *D.hdr;
+  const PassHdr hdr = inl.nscans >= 0 ? inl.hdr : *D.hdr;
   const int P = hdr.total;
-  uint32_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per-thread counts
-  const int which[8] = {ST_VALID, ST_SKIPPED, ST_BLOCK_CELLS, ST_UPDATES, ST_TRUNC_VOX, ST_TRUNC_BLK,
-                        ST_INSERT_FAIL, ST_UPD_DROPPED};
+  if (blockIdx.x == 0) {
+    // the previous pass's fold is over: clear its counters; store the
+    // inline descriptors for k_emit
+    const int t = threadIdx.x;
+    if (t == 0) D.ctr->fold_next = 0;
+    if (t < FC_COUNT) D.ctr->fold_n[t] = 0;
+    if (inl.nscans >= 0) {
+      if (t == 0) *const_cast<PassHdr*>(D.hdr) = inl.hdr;
+      for (int k = t; k < inl.nscans; k += blockDim.x) const_cast<ScanDev*>(D.scans)[k] = inl.scans[k];
+    }
+  }
+  // per-thread counts: ST_VALID, ST_SKIPPED, ST_BLOCK_CELLS, ST_UPDATES,
+  // ST_TRUNC_VOX, ST_TRUNC_BLK, ST_INSERT_FAIL, ST_UPD_DROPPED (the columns of
+  // the SR_BIN rows)
+  uint32_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // a new entry joins the workgroup's list, which goes to the pass's
   // new-entry list with one atomic at the end of the workgroup's points
   // (heap blocks are assigned by k_pass_scan), or joins the pass's list at
@@ -494,7 +570,7 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D) {
     }
     int nbc = 0;
     if (i < P) {
-      sd = D.scans[find_scan(D.scans, hdr.nscans, i)];
+      sd = scan_of(D, inl, hdr.nscans, i);
       m = D.maps[sd.map];
       const float4 q = hdr.points[i];
       p[0] = q.x;
@@ -651,7 +727,7 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D) {
     }
     __syncthreads();
   }
-  flush_stats(D, which, st);
+  write_stat_row(D, SR_BIN, st);
 }
 
 // ---- exclusive scan over a device-sized list -------------------------------------
@@ -808,8 +884,8 @@ __global__ void __launch_bounds__(256) k_pass_scan(Dev D) {
     if (threadIdx.x == 0) D.tile_sums[t] = tot;
   }
   // heap failures include quota failures, which are also counted apart
+  // (columns of the SR_SCAN rows: ST_NEW_BLOCKS, ST_HEAP_FAIL, ST_QUOTA_FAIL)
   uint32_t st[3] = {0, 0, 0};
-  const int which[3] = {ST_NEW_BLOCKS, ST_HEAP_FAIL, ST_QUOTA_FAIL};
   const int nn = ld_acq(&D.ctr->new_n);
   const int nchunks = (nn + 255) / 256;
   for (int c = gridDim.x - 1 - blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -820,7 +896,7 @@ __global__ void __launch_bounds__(256) k_pass_scan(Dev D) {
     st[1] += (r == kAssignHeapFail || r == kAssignQuotaFail) ? 1 : 0;
     st[2] += r == kAssignQuotaFail ? 1 : 0;
   }
-  flush_stats(D, which, st);
+  write_stat_row(D, SR_SCAN, st);
 }
 
 // Lanes of one wave exchange data through LDS: the wave's LDS requests
@@ -875,6 +951,12 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
   const PassHdr hdr = *D.hdr;
   const int P = hdr.total;
   const int64_t stride = D.pcap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // k_pass_scan and k_seg_apply, their last readers, are over
+    D.ctr->touched_n = 0;
+    D.ctr->new_n = 0;
+    D.ctr->fail_n = 0;
+  }
   // run rows that exist in the pair arrays (they hold hdr.stride rows)
   const int rows = min(kEmitRuns, hdr.stride);
   const bool wide = P > kEmitStagePoints;  // a pass of more than two 131k-point scans
@@ -1090,7 +1172,8 @@ __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
   const int nf = ld_acq(&D.ctr->fail_n);
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nf; q += gridDim.x * blockDim.x)
     D.keys[D.fail_list[q]] = kEmptyKey;
-  stat_add(D, ST_UPD_DROPPED, dropped);
+  const uint32_t row[1] = {(uint32_t)dropped};
+  write_stat_row(D, SR_SEG, row);
 }
 
 // Blocks of at most G records, 64/G blocks per wave: each group of G lanes
@@ -1667,7 +1750,8 @@ __global__ void __launch_bounds__(256, 7) k_fold_a(Dev D) {
       }
     }
   }
-  stat_add(D, ST_UNIQUE, uniq);
+  const uint32_t row[1] = {(uint32_t)uniq};
+  write_stat_row(D, SR_FOLD_A, row);
 }
 
 // Fold, second launch: blocks of kCountTileS+1..kCountTile records, and the
@@ -1721,7 +1805,8 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
                                                           uniq);
     }
   }
-  stat_add(D, ST_UNIQUE, uniq);
+  const uint32_t row[1] = {(uint32_t)uniq};
+  write_stat_row(D, SR_FOLD_B, row);
 }
 
 // ---- publish ---------------------------------------------------------------------------

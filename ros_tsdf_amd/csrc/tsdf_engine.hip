@@ -189,6 +189,8 @@ struct tsdf_engine {
   uint8_t* new_map = nullptr;
   int4* fold_desc = nullptr;
   unsigned long long* part = nullptr;
+  uint32_t* srows = nullptr;
+  StatGrids last_grids{};  // the grids of the last pass's kernels, by statistics region
   int64_t fold_cap = 0;
   int64_t fold_base[8] = {};
   uint8_t* touched_map = nullptr;
@@ -255,6 +257,7 @@ struct tsdf_engine {
     D.new_map = new_map;
     D.fold_desc = fold_desc;
     D.part = part;
+    D.srows = srows;
     D.fold_cap = fold_cap;
     for (int c = 0; c < 8; ++c) D.fold_base[c] = fold_base[c];
     D.touched_map = touched_map;
@@ -268,6 +271,7 @@ struct tsdf_engine {
 
 struct tsdf_batch_graph {
   tsdf_engine* eng = nullptr;
+  StatGrids grids{};  // the captured pass's grids
   // the graph's maps by slot and id: a launch checks that each map it uses
   // still exists (a destroyed map's slot is either empty or holds a map with
   // another id), and never touches a tsdf_map that may have been freed
@@ -454,10 +458,10 @@ int stage_descs(tsdf_engine* E, const PassHdr& hdr, const ScanDev* scans, int ns
 int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int tid, const PassInline& inl) {
   hipStream_t st = E->stream;
   const tsdf_launch_config& L = E->launch;
-  hipLaunchKernelGGL(k_pass_begin, dim3(64), dim3(256), 0, st, D, kPassStats, inl);
   CHK(timing_mark(E, tid, 0));
   const int tb = L.traverse_block;
-  hipLaunchKernelGGL(k_bin, dim3(grid_for(P, tb, L.traverse_grid_cap)), dim3(tb), 0, st, D);
+  const int bg = grid_for(P, tb, L.traverse_grid_cap);
+  hipLaunchKernelGGL(k_bin, dim3(bg), dim3(tb), 0, st, D, inl);
   CHK(timing_mark(E, tid, 1));
   // (heap assignment runs in k_pass_scan, inside the scan stage: the assign stage is empty)
   CHK(timing_mark(E, tid, 2));
@@ -472,12 +476,19 @@ int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int ti
   else hipLaunchKernelGGL(k_emit<false>, dim3(grid_for(P, 256, 65536)), dim3(256), 0, st, D);
   CHK(timing_mark(E, tid, 4));
   const int fg = std::max(1, L.fold_grid);
+  constexpr int kFoldBGrid = 512;
   with_state(E->state_format, [&](auto s) {
     using State = decltype(s);
     hipLaunchKernelGGL(k_fold_a<State>, dim3(fg), dim3(256), 0, st, D);
-    hipLaunchKernelGGL(k_fold_b<State>, dim3(512), dim3(kFoldBThreads), 0, st, D);
+    hipLaunchKernelGGL(k_fold_b<State>, dim3(kFoldBGrid), dim3(kFoldBThreads), 0, st, D);
     return 0;
   });
+  static_assert(kFoldBGrid <= kStatRows[SR_FOLD_B], "a statistics row per workgroup");
+  E->last_grids.n[SR_BIN] = bg;
+  E->last_grids.n[SR_SCAN] = sg;
+  E->last_grids.n[SR_SEG] = sg;
+  E->last_grids.n[SR_FOLD_A] = fg;
+  E->last_grids.n[SR_FOLD_B] = kFoldBGrid;
   CHK(timing_mark(E, tid, 5));
   HIPCHK(hipGetLastError());
   return TSDF_OK;
@@ -800,7 +811,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
       (r = dev_alloc(&E->touched, entries)) ||
       (r = dev_alloc(&E->tile_sums, entries / kScanTile + 2)) || (r = dev_alloc(&E->fail_list, entries)) ||
       (r = dev_alloc(&E->new_list, entries)) || (r = dev_alloc(&E->new_map, entries)) ||
-      (r = dev_alloc(&E->touched_map, entries)) || (r = dev_alloc(&E->part, ST_COUNT * kPartSlots)) ||
+      (r = dev_alloc(&E->touched_map, entries)) || (r = dev_alloc(&E->part, ST_COUNT * kPartSlots)) || (r = dev_alloc(&E->srows, (int64_t)kStatRowsTotal * kStatCols)) ||
       (r = dev_alloc(&E->sel, entries)) || (r = dev_alloc(&E->pub_cnt, entries + 1)) ||
       (r = dev_alloc(&E->pub_off, entries + 1)))
     return cleanup(r);
@@ -829,6 +840,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
       hipMemsetAsync(E->heap, 0, (size_t)vbytes, st) != hipSuccess ||
       hipMemsetAsync(E->blk_cnt, 0, sizeof(unsigned long long) * entries, st) != hipSuccess ||
       hipMemsetAsync(E->part, 0, sizeof(unsigned long long) * ST_COUNT * kPartSlots, st) != hipSuccess ||
+      hipMemsetAsync(E->srows, 0, sizeof(uint32_t) * kStatRowsTotal * kStatCols, st) != hipSuccess ||
       hipMemsetAsync(E->keys, 0xFF, sizeof(unsigned long long) * entries, st) != hipSuccess ||
       hipMemsetAsync(E->vals, 0xFF, sizeof(int32_t) * entries, st) != hipSuccess ||
       hipMemsetAsync(E->next, 0xFF, sizeof(int32_t) * c.total_buckets, st) != hipSuccess)
@@ -857,7 +869,7 @@ int tsdf_engine_destroy(tsdf_engine* E) {
     }
   void* ptrs[] = {E->maps_dev, E->mctr, E->ctr, E->keys, E->vals, E->blk_cnt, E->blk_base, E->next, E->ovf_stack,
                   E->heap, E->heap_free, E->freed, E->desc_dev, E->pair_e, E->pair_r, E->pair_n,
-                  E->seg, E->touched, E->tile_sums, E->fail_list, E->new_list, E->new_map, E->fold_desc, E->part,
+                  E->seg, E->touched, E->tile_sums, E->fail_list, E->new_list, E->new_map, E->fold_desc, E->part, E->srows,
                   E->touched_map, E->sel,
                   E->pub_cnt, E->pub_off, E->pub_buf, E->in_buf};
   for (void* p : ptrs)
@@ -1158,7 +1170,7 @@ int tsdf_process_scan(tsdf_map* m, const float* points_host, int64_t n, const fl
   if (!r) {
     // the pass's failure counters, reduced while the stream is still in order
     // (publish and gc keep their own statistics)
-    hipLaunchKernelGGL(k_stats_reduce, dim3(ST_COUNT), dim3(256), 0, E->stream, E->view(), kDropStats);
+    hipLaunchKernelGGL(k_pass_stats, dim3(ST_COUNT), dim3(256), 0, E->stream, E->view(), E->last_grids, kDropStats);
     e = hipGetLastError();
     if (e != hipSuccess) r = fail(TSDF_ERR_HIP, "statistics reduction failed: %s", hipGetErrorString(e));
   }
@@ -1179,9 +1191,9 @@ int tsdf_process_scan(tsdf_map* m, const float* points_host, int64_t n, const fl
 int tsdf_last_stats(tsdf_engine* E, tsdf_scan_stats* out) {
   if (!E || !out) return fail(TSDF_ERR_INVALID_ARG, "null argument");
   CHK(set_device(E));
-  // the last pass's statistics stay in their partial slots until the next
-  // pass starts: reduce them now
-  hipLaunchKernelGGL(k_stats_reduce, dim3(ST_COUNT), dim3(256), 0, E->stream, E->view(), kPassStats);
+  // the last pass's statistics stay in their workgroup rows until the next
+  // pass overwrites them: sum them now
+  hipLaunchKernelGGL(k_pass_stats, dim3(ST_COUNT), dim3(256), 0, E->stream, E->view(), E->last_grids, kPassStats);
   HIPCHK(hipGetLastError());
   CHK(read_ctr(E));
   const unsigned long long* s = E->ctr_host->stats;
@@ -1211,8 +1223,9 @@ int tsdf_engine_set_launch(tsdf_engine* E, const tsdf_launch_config* cfg) {
   if (!E || !cfg) return fail(TSDF_ERR_INVALID_ARG, "null argument");
   if (cfg->traverse_block < 64 || cfg->traverse_block > 256 || (cfg->traverse_block % 64) != 0)
     return fail(TSDF_ERR_INVALID_ARG, "traverse_block must be 64, 128, 192 or 256");
-  if (cfg->traverse_grid_cap < 1 || cfg->fold_grid < 1 || cfg->fold_grid > 65536)
-    return fail(TSDF_ERR_INVALID_ARG, "grid sizes out of range");
+  if (cfg->traverse_grid_cap < 1 || cfg->traverse_grid_cap > kStatRows[SR_BIN] || cfg->fold_grid < 1 ||
+      cfg->fold_grid > kStatRows[SR_FOLD_A])
+    return fail(TSDF_ERR_INVALID_ARG, "grid sizes out of range (1..65536)");
   E->launch = *cfg;
   return TSDF_OK;
 }
@@ -1295,6 +1308,7 @@ static int graph_capture(tsdf_batch_graph* g) {
   }
   if (e != hipSuccess) return fail(TSDF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
   g->graph = graph;
+  g->grids = E->last_grids;
   HIPCHK(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
   g->gen = E->scratch_gen;
   return TSDF_OK;
@@ -1372,6 +1386,7 @@ int tsdf_batch_graph_launch(tsdf_batch_graph* g, const tsdf_scan_desc* scans, in
   CHK(stage_descs(E, hdr, sd, nscans));
   HIPCHK(hipGraphLaunch(g->exec, E->stream));
   E->last_points = P;
+  E->last_grids = g->grids;
   return TSDF_OK;
 }
 
