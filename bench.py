@@ -15,7 +15,9 @@ Also reported on the same JSON line:
                against the FP64 oracle
   serving      Poisson arrivals over 16 maps at 400 scans/s: continuous
                batching vs plain per-scan serving, p50/p99 latency
-  roofline     the dominant kernel against MI355X HBM bandwidth
+  roofline     the dominant kernel against MI355X HBM bandwidth; its VALU
+               issue rate; k_bin's global atomics against the device's
+               atomic rate, measured by the calibration kernels in this run
   cpu_baseline the oracle's multithreaded CPU path over the whole dense workload
 """
 from __future__ import annotations
@@ -113,6 +115,40 @@ def full_size_batches(eng, bmaps_pts, bmaps_sens, knobs):
     fbuf.free()
     return {"points_per_s": fpts / tfb, "plain_per_scan_points_per_s": fpts / tfp, "speedup_vs_plain": tfp / tfb,
             "scans_per_launch": per, "points_per_scan": fpts / (2 * per), "ms_per_launch": 1e3 * tfb / 2}
+
+
+def atomics_line(stage_ms):
+    """k_bin's global atomics against the device's atomic rate.  The count per launch is a stored profile
+    (rocprofv3 TCC_ATOMIC_sum over the plain workload, profiles/atomics_dense.json); its duration is this
+    run's (the traverse stage's HIP events bracket k_bin alone); the peak is measured now by the calibration
+    kernels: 64-bit adds with return, one 128-byte line per lane, the pattern of k_bin's per-block record
+    reservations."""
+    from ros_tsdf_amd import calib
+    try:
+        rates = calib.atomic_rates()
+    except Exception as e:  # the calibration library is part of the build: report, do not hide
+        return {"error": "calibration failed: %s" % e}
+    peak = rates["scatter_ret64"]["ops_per_s"]
+    out = {"kernel": "k_bin", "unit": "G L2 atomics/s", "peak": peak / 1e9,
+           "peak_pattern": "scatter_ret64 (calibration kernel, this run)",
+           "calibration_G_per_s": {k: v["ops_per_s"] / 1e9 for k, v in rates.items()},
+           "same_address_ns": rates["same_ret32"]["ns_per_op"]}
+    apath = os.path.join(ROOT, "profiles", "atomics_dense.json")
+    try:
+        with open(apath) as f:
+            prof = json.load(f)["dense_plain"]
+    except Exception:
+        prof = None
+    if prof and "k_bin" in prof and stage_ms.get("traverse"):
+        n = prof["k_bin"]["l2_atomics"]
+        rate = n / (stage_ms["traverse"] * 1e-3)
+        out.update({"l2_atomics_per_launch": n, "atomic_wave_insts_per_launch": prof["k_bin"]["atomic_wave_insts"],
+                    "lds_atomic_wave_insts_per_launch": prof["k_bin"]["lds_atomic_wave_insts"],
+                    "achieved": rate / 1e9, "frac": rate / peak,
+                    "pass_l2_atomics_per_launch": sum(v["l2_atomics"] for v in prof.values()),
+                    "counts_source": "profiles/atomics_dense.json: rocprofv3 --pmc TCC_ATOMIC_sum over this workload; "
+                                     "a stored profile, not counted in this run"})
+    return out
 
 
 def main():
@@ -244,6 +280,7 @@ def main():
     roofline["copy_kernel_GBps"] = copy_gbps
     csrc.free()
     cdst.free()
+    roofline["atomics"] = atomics_line(stage_ms)
     plain_lat = plain_lat_pinned = None
     if extras:
         # ------------- full per-scan call latency (reference sequence) -------------

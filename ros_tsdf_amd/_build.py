@@ -16,6 +16,7 @@ INCLUDE = os.path.join(ROOT, "include")
 
 ENGINE_SO = os.path.join(LIB_DIR, "libtsdf_hip.so")
 PACKAGE_SO = os.path.join(LIB_DIR, "libtsdf_package.so")
+CALIB_SO = os.path.join(LIB_DIR, "libtsdf_calib.so")
 HANDLER_TEST_BIN = os.path.join(LIB_DIR, "handler_dropin_test")
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -45,6 +46,15 @@ def build_engine() -> str:
     return ENGINE_SO
 
 
+def build_calib() -> str:
+    """The bench's device-rate calibration kernels (include/tsdf_calib.h)."""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = CALIB_SO + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-I", INCLUDE, "-o", tmp, os.path.join(CSRC, "calib.hip")])
+    os.replace(tmp, CALIB_SO)
+    return CALIB_SO
+
+
 def build_package() -> str:
     """The C++ drop-in (tsdf::TSDFHandler & co.) over the C ABI, plus its test program."""
     os.makedirs(LIB_DIR, exist_ok=True)
@@ -65,6 +75,7 @@ def build_package() -> str:
 def build_all() -> None:
     build_engine()
     build_package()
+    build_calib()
 
 
 if __name__ == "__main__":

@@ -91,3 +91,19 @@ def test_layouts():
     assert C.sizeof(abi.Request) == 32
     assert C.sizeof(abi.ScanStats) == 16 * 8
     assert np.dtype(np.float32).itemsize * 4 == 16  # pcl::PointXYZ rows
+
+
+def test_calibration_library_exports_its_entry_points():
+    from ros_tsdf_amd import calib
+    src = open(os.path.join(ROOT, "include", "tsdf_calib.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(tsdf_[a-z0-9_]+)\s*\(", src)))
+    assert names == ["tsdf_calib_atomics", "tsdf_calib_last_error"]
+    L = calib.load()
+    for n in names:
+        assert hasattr(L, n), n
+    # arguments are checked before any device call
+    ops, secs = C.c_int64(), C.c_double()
+    assert L.tsdf_calib_atomics(0, 9, 1, 1, C.byref(ops), C.byref(secs)) == -1
+    assert b"unknown pattern" in L.tsdf_calib_last_error()
+    assert L.tsdf_calib_atomics(0, 0, 0, 1, C.byref(ops), C.byref(secs)) == -1
