@@ -71,7 +71,7 @@ def plain_throughput(eng, m, buf, pool, idx, with_stats=True):
 def full_size_batches(eng, bmaps_pts, bmaps_sens, knobs):
     """Batched against plain per-scan on full-size scans (131,072 points each): 8 maps x 4 scans per launch,
     three launches (the first untimed).  knobs: traverse_block, fold_grid, buckets_log2, max_chain."""
-    from ros_tsdf_amd import BatchGraph, configs
+    from ros_tsdf_amd import BatchGraph, autotune, configs
 
     def mk():
         return [eng.map(configs.DENSE_PARAMS, buckets_log2=knobs["buckets_log2"], max_chain=knobs["max_chain"])
@@ -85,7 +85,7 @@ def full_size_batches(eng, bmaps_pts, bmaps_sens, knobs):
     per = len(fscans) // 3
     fdescs = [[(fpool.maps[j], fpool.counts[j], fpool.offsets[j], fpool.sensors[j]) for j in range(per * b, per * b + per)]
               for b in range(3)]
-    eng.set_launch(traverse_block=knobs["traverse_block"], fold_grid=knobs["fold_grid"])
+    eng.set_launch(**autotune.launch_kwargs(knobs))
     maps = mk()
     g = BatchGraph(eng, maps, max_points=max(sum(x[1] for x in d) for d in fdescs), max_scans=per)
     g.launch(fdescs[0], fbuf)

@@ -235,12 +235,17 @@ int tsdf_batch_graph_create(tsdf_engine* eng, tsdf_map* const* maps, int32_t nma
 int tsdf_batch_graph_launch(tsdf_batch_graph* g, const tsdf_scan_desc* scans, int32_t nscans, const float* points_dev);
 int tsdf_batch_graph_destroy(tsdf_batch_graph* g);
 
-/* Launch-shape knobs of the batched pipeline, set by the autotuner. */
+/* Launch-shape knobs of the pass, set by the autotuner.  A traversal chunk
+ * is rays_per_lane x traverse_block consecutive points sharing one LDS block
+ * table, i.e. 64 x rays_per_lane rays per wavefront. */
 typedef struct {
   int32_t traverse_block;    /* threads per workgroup of the traversal kernel: 64, 128, 192 or 256 */
   int32_t traverse_grid_cap; /* max workgroups of the traversal kernel (1..65536) */
   int32_t fold_grid;         /* workgroups of the first fold launch, 1..65536 (its work items are spread over them) */
-  int32_t reserved;
+  int32_t rays_per_lane;     /* rays each traversal lane walks per chunk: 1..4 (0 reads as 1) */
+  int32_t traverse_queue;    /* 1: the traversal kernel runs as a persistent grid of resident workgroups that take
+                                chunks from a device work queue; 0: one workgroup per chunk up to traverse_grid_cap */
+  int32_t reserved;          /* 0 */
 } tsdf_launch_config;
 int tsdf_engine_set_launch(tsdf_engine* eng, const tsdf_launch_config* cfg);
 int tsdf_engine_get_launch(tsdf_engine* eng, tsdf_launch_config* cfg);

@@ -70,7 +70,7 @@ class ScanDesc(C.Structure):
 
 class LaunchConfig(C.Structure):
     _fields_ = [("traverse_block", C.c_int32), ("traverse_grid_cap", C.c_int32), ("fold_grid", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("rays_per_lane", C.c_int32), ("traverse_queue", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Request(C.Structure):

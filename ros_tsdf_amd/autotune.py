@@ -4,8 +4,12 @@ It measures the batched pipeline on a sample workload and picks the launch shape
 highest throughput. The sample is a list of batches, each a list of scans of independent maps.
 
 Knobs:
-  traverse_block  threads per workgroup of the binning kernel (64, 128, 256): one ray per lane, so this is the
-                  number of rays whose blocks a workgroup gathers in its LDS table before touching the hash
+  traverse_block  threads per workgroup of the binning kernel (64, 128, 256)
+  rays_per_lane   rays each lane of the binning kernel walks per chunk (1, 2, 4), i.e. 64, 128 or 256 rays per
+                  wavefront: a chunk of traverse_block x rays_per_lane rays gathers its blocks in one LDS table
+                  before touching the hash
+  traverse_queue  0: one binning workgroup per chunk; 1: a persistent grid of resident workgroups taking chunks
+                  from a device work queue
   fold_grid       workgroups of the first fold launch (work items are spread over them)
   buckets_log2    home buckets per map; a bucket holds 8 entries, so 17..21 spans 2^20..2^24 entries
   max_chain       longest overflow chain a map may grow (overflow-list length)
@@ -26,11 +30,20 @@ from .engine import BatchGraph, Engine
 
 SPACE = {
     "traverse_block": (64, 128, 256),
+    "rays_per_lane": (1, 2, 4),
+    "traverse_queue": (0, 1),
     "fold_grid": (2048, 4096, 8192, 16384),
     "buckets_log2": (17, 18, 19, 20, 21),
     "max_chain": (4, 16, 64),
 }
-DEFAULT = {"traverse_block": 256, "fold_grid": 4096, "buckets_log2": 17, "max_chain": 64}
+DEFAULT = {"traverse_block": 256, "rays_per_lane": 1, "traverse_queue": 0, "fold_grid": 4096, "buckets_log2": 17,
+           "max_chain": 64}
+LAUNCH_KNOBS = ("traverse_block", "rays_per_lane", "traverse_queue", "fold_grid")
+
+
+def launch_kwargs(cfg: dict) -> dict:
+    """The launch-shape knobs of a configuration, as Engine.set_launch arguments (absent ones at DEFAULT)."""
+    return {k: cfg.get(k, DEFAULT[k]) for k in LAUNCH_KNOBS}
 
 
 def measure(engine: Engine, params, nmaps: int, batches, points, cfg: dict, reps: int = 1) -> dict:
@@ -40,7 +53,7 @@ def measure(engine: Engine, params, nmaps: int, batches, points, cfg: dict, reps
     points: a DeviceBuffer holding every batch's points.
     Returns a dict with points_per_s, ms_per_batch and failures (the summed failure counters).
     """
-    engine.set_launch(traverse_block=cfg["traverse_block"], fold_grid=cfg["fold_grid"])
+    engine.set_launch(**launch_kwargs(cfg))
     best = None
     failures = 0
     max_pts = max(sum(s[1] for s in b) for b in batches)
@@ -118,5 +131,5 @@ def tune(engine: Engine, params, nmaps: int, batches, points, space: dict | None
             r = run(cand)
             if score(r) > score(best_r):
                 best_r, cur = r, cand
-    engine.set_launch(traverse_block=cur["traverse_block"], fold_grid=cur["fold_grid"])
+    engine.set_launch(**launch_kwargs(cur))
     return cur, history

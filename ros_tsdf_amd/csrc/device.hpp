@@ -70,7 +70,7 @@ struct EngCtr {
   int32_t freed_n;    // blocks freed by the current gc
   int32_t new_n;      // entries inserted in this pass, waiting for heap blocks
   int32_t fold_next;  // large-block fold items handed out beyond the first per workgroup
-  int32_t pad;
+  int32_t bin_next;   // k_bin ray chunks handed out from the work queue beyond the first per workgroup
   int32_t fold_n[8];  // touched blocks per fold size class
   unsigned long long stats[ST_COUNT];
 };

@@ -286,9 +286,10 @@ __device__ inline int find_or_insert(const Dev& D, const MapDev& m, unsigned lon
 // (larger batches, and graph replays, whose captured arguments would be
 // stale).
 // The pass counters need no launch to clear them either: each is cleared
-// after its last reader, by one thread of a later kernel (touched_n, new_n
-// and fail_n by k_emit, the fold counters by the next pass's k_bin), and
-// pass statistics are per-workgroup rows that every pass overwrites.
+// after its last reader, by one thread of a later kernel (touched_n, new_n,
+// fail_n and bin_next by k_emit, the fold counters by the next pass's
+// k_bin), and pass statistics are per-workgroup rows that every pass
+// overwrites.
 constexpr int kInlineScans = 32;
 struct PassInline {
   PassHdr hdr;
@@ -467,10 +468,25 @@ __device__ __forceinline__ int peek_find(const HomePeek& h, unsigned long long k
   return e;
 }
 
+// How k_bin's workgroups take the pass's rays.  A chunk is rays_per_lane x
+// blockDim consecutive points, which share one LDS block table (more rays
+// per table: more of their blocks deduplicated before the global lookups
+// and record reservations, fewer workgroups).  Workgroup w takes chunk w,
+// then chunk w + gridDim (queue == 0), or the next one from the pass's work
+// queue (queue == 1: a grid of resident workgroups, persistent for the
+// launch, each taking chunks until none is left).
+constexpr int kMaxRaysPerLane = 4;
+struct BinSched {
+  int32_t rays_per_lane;  // 1..kMaxRaysPerLane
+  int32_t queue;
+};
+
 // 168 VGPRs at most: three waves per SIMD where the grid has them (batched
 // passes; a 131k-point scan launches two per SIMD whatever the budget), with
-// a handful of spills off the walks.
-__global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl) {
+// a handful of spills off the walks.  kMulti: rays_per_lane may exceed 1
+// (one ray per lane compiles without the per-ray loops and their registers).
+template <bool kMulti>
+__global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched bs) {
   __shared__ unsigned long long tkey[kBinSlots];
   __shared__ int tent[kBinSlots];
   __shared__ uint32_t trec[kBinSlots];
@@ -480,7 +496,7 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl) {
   __shared__ uint32_t ltouch[kBinSlots];
   __shared__ uint8_t ltouch_map[kBinSlots];
   __shared__ uint16_t locc[kBinSlots];
-  __shared__ int n_new, n_touch, g_touch, g_new, n_occ;
+  __shared__ int n_new, n_touch, g_touch, g_new, n_occ, next_chunk;
   const PassHdr hdr = inl.nscans >= 0 ? inl.hdr : *D.hdr;
   const int P = hdr.total;
   if (blockIdx.x == 0) {
@@ -540,7 +556,11 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl) {
     }
   };
   const int64_t stride = D.pcap;
-  for (int base = blockIdx.x * blockDim.x; base < P; base += gridDim.x * blockDim.x) {
+  const int R = kMulti ? bs.rays_per_lane : 1;
+  const int chunk = R * (int)blockDim.x;
+  const int nchunks = (P + chunk - 1) / chunk;
+  for (int c = blockIdx.x; c < nchunks;) {
+    const int base = c * chunk;
     for (int x = threadIdx.x; x < kBinSlots; x += blockDim.x) {
       tkey[x] = kEmptyKey;
       trec[x] = 0;
@@ -552,24 +572,14 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl) {
     }
     __syncthreads();
     // ---- phase A1: block walks; blocks gathered in the LDS table ----
-    const int i = base + threadIdx.x;
-    int nj = 0;
+    // The ray of sub-step r is point base + r * blockDim + threadIdx.x.  Its
+    // scan, map, segment and validity come from setup(); the last ray's stay
+    // in registers for its voxel walk, the others' are set up again there.
     bool walk = false;
     ScanDev sd;
     MapDev m;
     float p[3], s[3], a[3], b[3];
-    // the block walk's first kBlkReg cells and their table slots (-1: not in
-    // the table), so that the voxel walk's runs find their slots without
-    // probing the table
-    int32_t bk0[kBlkReg], bk1[kBlkReg], bk2[kBlkReg];
-    int bsl[kBlkReg];
-#pragma unroll
-    for (int u = 0; u < kBlkReg; ++u) {
-      bk0[u] = bk1[u] = bk2[u] = 0;
-      bsl[u] = -1;
-    }
-    int nbc = 0;
-    if (i < P) {
+    auto setup = [&](int i) {
       sd = scan_of(D, inl, hdr.nscans, i);
       m = D.maps[sd.map];
       const float4 q = hdr.points[i];
@@ -579,35 +589,53 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl) {
       s[0] = sd.s[0];
       s[1] = sd.s[1];
       s[2] = sd.s[2];
-      if (!point_valid(p, s, m.vs)) {
-        st[1]++;
-      } else {
-        st[0]++;
-        walk = true;
-        truncation_segment(p, s, m.T, a, b);
-        bool tb = false;
-        const int nb = walk_cells(a, b, m.BL, m.sc_b, m.isc_b, m.half_BL, m.max_blk, tb, [&](int32_t k0, int32_t k1, int32_t k2) {
-          const int32_t kb[3] = {k0, k1, k2};
-          const int slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb));
-          if (slot < 0) {
-            bool ins;
-            const int e = find_or_insert(D, m, pack_key(k0, k1, k2), ins);
-            if (e < 0) st[6]++;
-            else if (ins) note_new(e, m.slot);
-          }
+      walk = point_valid(p, s, m.vs);
+      if (walk) truncation_segment(p, s, m.T, a, b);
+    };
+    // the last ray's first kBlkReg block-walk cells and their table slots
+    // (-1: not in the table), so that its voxel walk's runs find their slots
+    // without probing the table
+    int32_t bk0[kBlkReg], bk1[kBlkReg], bk2[kBlkReg];
+    int bsl[kBlkReg];
+    for (int r = 0; r < R; ++r) {
+      const int i = base + r * (int)blockDim.x + (int)threadIdx.x;
 #pragma unroll
-          for (int u = 0; u < kBlkReg; ++u)
-            if (u == nbc) {
-              bk0[u] = k0;
-              bk1[u] = k1;
-              bk2[u] = k2;
-              bsl[u] = slot;
+      for (int u = 0; u < kBlkReg; ++u) {
+        bk0[u] = bk1[u] = bk2[u] = 0;
+        bsl[u] = -1;
+      }
+      int nbc = 0;
+      walk = false;
+      if (i < P) {
+        setup(i);
+        if (!walk) {
+          st[1]++;
+        } else {
+          st[0]++;
+          bool tb = false;
+          const int nb = walk_cells(a, b, m.BL, m.sc_b, m.isc_b, m.half_BL, m.max_blk, tb, [&](int32_t k0, int32_t k1, int32_t k2) {
+            const int32_t kb[3] = {k0, k1, k2};
+            const int slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb));
+            if (slot < 0) {
+              bool ins;
+              const int e = find_or_insert(D, m, pack_key(k0, k1, k2), ins);
+              if (e < 0) st[6]++;
+              else if (ins) note_new(e, m.slot);
             }
-          ++nbc;
-          return true;
-        });
-        st[2] += nb;
-        st[5] += tb ? 1 : 0;
+#pragma unroll
+            for (int u = 0; u < kBlkReg; ++u)
+              if (u == nbc) {
+                bk0[u] = k0;
+                bk1[u] = k1;
+                bk2[u] = k2;
+                bsl[u] = slot;
+              }
+            ++nbc;
+            return true;
+          });
+          st[2] += nb;
+          st[5] += tb ? 1 : 0;
+        }
       }
     }
     __syncthreads();
@@ -630,63 +658,81 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl) {
       key0 = table_map_key(tkey[locc[threadIdx.x]], ms);
       pk0 = peek_home(D, D.maps[ms], key0);
     }
-    // ---- phase A2: voxel walks ----
-    if (walk) {
-      // A run (the voxels of one block, consecutive along the walk) gets
-      // its record offset in the block's LDS slot, or, past a full table,
-      // straight from the global counter.  A block the block walk did not
-      // list joins the table's list here.
-      auto close_run = [&](const int32_t kb[3], int rn) {
-        int slot = -1;
+    // ---- phase A2: voxel walks, the last ray's first ----
+    int njr[kMaxRaysPerLane];
+    for (int q = 0; q < R; ++q) {
+      const int r = q == 0 ? R - 1 : q - 1;
+      const int i = base + r * (int)blockDim.x + (int)threadIdx.x;
+      if (q == 1) {
+        // the cached cells were the last ray's
 #pragma unroll
-        for (int u = 0; u < kBlkReg; ++u)
-          if (bsl[u] >= 0 && bk0[u] == kb[0] && bk1[u] == kb[1] && bk2[u] == kb[2]) slot = bsl[u];
-        bool added = false;
-        if (slot < 0) slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb), &added);
-        if (slot >= 0) {
-          if (added) locc[atomicAdd(&n_occ, 1)] = (uint16_t)slot;
-          const uint32_t local = atomicAdd(&trec[slot], (uint32_t)rn);
-          D.pair_e[(int64_t)nj * stride + i] = (uint32_t)slot;
-          D.pair_r[(int64_t)nj * stride + i] = local | ((uint32_t)rn << 24);
-        } else {
-          bool ins;
-          const int e = find_or_insert(D, m, pack_key(kb[0], kb[1], kb[2]), ins);
-          if (e < 0) {
-            st[6]++;
-            st[7] += rn;
-            D.pair_e[(int64_t)nj * stride + i] = kPairFail;
+        for (int u = 0; u < kBlkReg; ++u) bsl[u] = -1;
+      }
+      if (q > 0) {
+        walk = false;
+        if (i < P) setup(i);
+      }
+      int nj = 0;
+      if (walk) {
+        // A run (the voxels of one block, consecutive along the walk) gets
+        // its record offset in the block's LDS slot, or, past a full table,
+        // straight from the global counter.  A block the block walk did not
+        // list joins the table's list here.
+        auto close_run = [&](const int32_t kb[3], int rn) {
+          int slot = -1;
+#pragma unroll
+          for (int u = 0; u < kBlkReg; ++u)
+            if (bsl[u] >= 0 && bk0[u] == kb[0] && bk1[u] == kb[1] && bk2[u] == kb[2]) slot = bsl[u];
+          bool added = false;
+          if (slot < 0) slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb), &added);
+          if (slot >= 0) {
+            if (added) locc[atomicAdd(&n_occ, 1)] = (uint16_t)slot;
+            const uint32_t local = atomicAdd(&trec[slot], (uint32_t)rn);
+            D.pair_e[(int64_t)nj * stride + i] = (uint32_t)slot;
+            D.pair_r[(int64_t)nj * stride + i] = local | ((uint32_t)rn << 24);
           } else {
-            if (ins) note_new(e, m.slot);
-            const unsigned long long old = atomicAdd(&D.blk_cnt[e], (unsigned long long)rn << 32);
-            if (old == 0) {
-              const int t = atomicAdd(&D.ctr->touched_n, 1);
-              D.touched[t] = (uint32_t)e;
-              D.touched_map[t] = (uint8_t)m.slot;
+            bool ins;
+            const int e = find_or_insert(D, m, pack_key(kb[0], kb[1], kb[2]), ins);
+            if (e < 0) {
+              st[6]++;
+              st[7] += rn;
+              D.pair_e[(int64_t)nj * stride + i] = kPairFail;
+            } else {
+              if (ins) note_new(e, m.slot);
+              const unsigned long long old = atomicAdd(&D.blk_cnt[e], (unsigned long long)rn << 32);
+              if (old == 0) {
+                const int t = atomicAdd(&D.ctr->touched_n, 1);
+                D.touched[t] = (uint32_t)e;
+                D.touched_map[t] = (uint8_t)m.slot;
+              }
+              D.pair_e[(int64_t)nj * stride + i] = kPairDirect | (uint32_t)e;
+              D.pair_r[(int64_t)nj * stride + i] = (uint32_t)(old >> 32);
+              st[3] += rn;
             }
-            D.pair_e[(int64_t)nj * stride + i] = kPairDirect | (uint32_t)e;
-            D.pair_r[(int64_t)nj * stride + i] = (uint32_t)(old >> 32);
-            st[3] += rn;
           }
-        }
-        ++nj;
-      };
-      // The voxel walk notes only which cells enter a new block (walk_runs);
-      // the table work for its runs follows the walk, so the lanes of a wave
-      // do it together rather than at scattered steps of their walks.
-      bool tv = false;
-      walk_runs(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t b0, int32_t b1, int32_t b2, int rn) {
-        const int32_t kb[3] = {b0, b1, b2};
-        close_run(kb, rn);
-      });
-      st[4] += tv ? 1 : 0;
+          ++nj;
+        };
+        // The voxel walk notes only which cells enter a new block (walk_runs);
+        // the table work for its runs follows the walk, so the lanes of a wave
+        // do it together rather than at scattered steps of their walks.
+        bool tv = false;
+        walk_runs(a, b, m.vs, m.sc_v, m.isc_v, m.half_vs, m.max_vox, tv, [&](int32_t b0, int32_t b1, int32_t b2, int rn) {
+          const int32_t kb[3] = {b0, b1, b2};
+          close_run(kb, rn);
+        });
+        st[4] += tv ? 1 : 0;
+      }
+      if (i < P) D.pair_n[i] = (uint8_t)nj;
+#pragma unroll
+      for (int u = 0; u < kMaxRaysPerLane; ++u)
+        if (u == r) njr[u] = nj;
     }
-    if (i < P) D.pair_n[i] = (uint8_t)nj;
     __syncthreads();
     // ---- phase B: one global lookup and one counter atomic per listed block
     // (the first per thread from the bucket read ahead) ----
-    for (int y = threadIdx.x, r = 0; y < n_occ; y += blockDim.x, ++r) {
+    for (int y = threadIdx.x, rr = 0; y < n_occ; y += blockDim.x, ++rr) {
       int e = -1;
-      if (y < n_listed && r == 0) e = peek_find(pk0, key0);
+      if (y < n_listed && rr == 0) e = peek_find(pk0, key0);
       resolve_slot(locc[y], e);
     }
     __syncthreads();
@@ -694,6 +740,7 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl) {
     if (threadIdx.x == 0) {
       g_touch = n_touch ? atomicAdd(&D.ctr->touched_n, n_touch) : 0;
       g_new = nn ? atomicAdd(&D.ctr->new_n, nn) : 0;
+      next_chunk = bs.queue ? (int)gridDim.x + atomicAdd(&D.ctr->bin_next, 1) : c + (int)gridDim.x;
     }
     __syncthreads();
     for (int x = threadIdx.x; x < n_touch; x += blockDim.x) {
@@ -705,7 +752,13 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl) {
       D.new_map[g_new + x] = lnew_map[x];
     }
     // ---- phase C: final (entry, record offset) of every run ----
-    if (i < P) {
+    for (int r = 0; r < R; ++r) {
+      const int i = base + r * (int)blockDim.x + (int)threadIdx.x;
+      int nj = 0;
+#pragma unroll
+      for (int u = 0; u < kMaxRaysPerLane; ++u)
+        if (u == r) nj = njr[u];
+      if (i >= P) continue;
       for (int j = 0; j < nj; ++j) {
         const uint32_t v = D.pair_e[(int64_t)j * stride + i];
         if (v == kPairFail || (v & kPairDirect)) {
@@ -725,6 +778,7 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl) {
         }
       }
     }
+    c = next_chunk;
     __syncthreads();
   }
   write_stat_row(D, SR_BIN, st);
@@ -956,6 +1010,7 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
     D.ctr->touched_n = 0;
     D.ctr->new_n = 0;
     D.ctr->fail_n = 0;
+    D.ctr->bin_next = 0;  // (k_bin's work queue)
   }
   // run rows that exist in the pair arrays (they hold hdr.stride rows)
   const int rows = min(kEmitRuns, hdr.stride);

@@ -216,7 +216,9 @@ struct tsdf_engine {
   int64_t last_points = 0;
 
   // launch knobs
-  tsdf_launch_config launch{256, 65536, 4096, 0};
+  tsdf_launch_config launch{256, 65536, 4096, 1, 0, 0};
+  int cus = 0;           // compute units of the device
+  int bin_resident = 0;  // k_bin workgroups of traverse_block threads resident at once on the device
 
   // timing
   bool timing = false;
@@ -460,8 +462,14 @@ int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int ti
   const tsdf_launch_config& L = E->launch;
   CHK(timing_mark(E, tid, 0));
   const int tb = L.traverse_block;
-  const int bg = grid_for(P, tb, L.traverse_grid_cap);
-  hipLaunchKernelGGL(k_bin, dim3(bg), dim3(tb), 0, st, D, inl);
+  BinSched bs;
+  bs.rays_per_lane = std::max(1, L.rays_per_lane);
+  bs.queue = L.traverse_queue;
+  // queue: the resident workgroups (each keeps taking chunks); else one
+  // workgroup per chunk up to the cap
+  const int bg = grid_for(P, (int64_t)tb * bs.rays_per_lane, bs.queue ? E->bin_resident : L.traverse_grid_cap);
+  if (bs.rays_per_lane > 1) hipLaunchKernelGGL(k_bin<true>, dim3(bg), dim3(tb), 0, st, D, inl, bs);
+  else hipLaunchKernelGGL(k_bin<false>, dim3(bg), dim3(tb), 0, st, D, inl, bs);
   CHK(timing_mark(E, tid, 1));
   // (heap assignment runs in k_pass_scan, inside the scan stage: the assign stage is empty)
   CHK(timing_mark(E, tid, 2));
@@ -491,6 +499,17 @@ int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int ti
   E->last_grids.n[SR_FOLD_B] = kFoldBGrid;
   CHK(timing_mark(E, tid, 5));
   HIPCHK(hipGetLastError());
+  return TSDF_OK;
+}
+
+// k_bin workgroups of `block` threads resident at once: per-CU occupancy
+// times the CU count (the persistent grid of the queue mode).
+int bin_residency(tsdf_engine* E, int block, int& out) {
+  int per_cu = 0;
+  // (both instances have the same register and LDS budget)
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_bin<true>), block, 0));
+  if (per_cu < 1) return fail(TSDF_ERR_HIP, "k_bin: no workgroup of %d threads fits a CU", block);
+  out = per_cu * E->cus;
   return TSDF_OK;
 }
 
@@ -796,8 +815,12 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
     tsdf_engine_destroy(E);
     return r;
   };
+  int r0;
   if (hipSetDevice(E->device) != hipSuccess) return cleanup(fail(TSDF_ERR_HIP, "hipSetDevice failed"));
   resolve_scratch(E->device);  // for tsdf_integrate_noalloc; a failure is reported by that call
+  if (hipDeviceGetAttribute(&E->cus, hipDeviceAttributeMultiprocessorCount, E->device) != hipSuccess || E->cus < 1)
+    return cleanup(fail(TSDF_ERR_HIP, "the compute-unit count of device %d is unavailable", E->device));
+  if ((r0 = bin_residency(E, E->launch.traverse_block, E->bin_resident)) != TSDF_OK) return cleanup(r0);
   if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(TSDF_ERR_HIP, "hipStreamCreate failed"));
   int r;
@@ -1226,7 +1249,16 @@ int tsdf_engine_set_launch(tsdf_engine* E, const tsdf_launch_config* cfg) {
   if (cfg->traverse_grid_cap < 1 || cfg->traverse_grid_cap > kStatRows[SR_BIN] || cfg->fold_grid < 1 ||
       cfg->fold_grid > kStatRows[SR_FOLD_A])
     return fail(TSDF_ERR_INVALID_ARG, "grid sizes out of range (1..65536)");
+  if (cfg->rays_per_lane < 0 || cfg->rays_per_lane > kMaxRaysPerLane)
+    return fail(TSDF_ERR_INVALID_ARG, "rays_per_lane must be 0..%d", kMaxRaysPerLane);
+  if (cfg->traverse_queue != 0 && cfg->traverse_queue != 1) return fail(TSDF_ERR_INVALID_ARG, "traverse_queue must be 0 or 1");
+  if (cfg->reserved != 0) return fail(TSDF_ERR_INVALID_ARG, "reserved must be 0");
+  CHK(set_device(E));
+  int resident = 0;
+  CHK(bin_residency(E, cfg->traverse_block, resident));
   E->launch = *cfg;
+  if (E->launch.rays_per_lane == 0) E->launch.rays_per_lane = 1;
+  E->bin_resident = resident;
   return TSDF_OK;
 }
 

@@ -154,14 +154,18 @@ class Engine:
         mp = (C.c_void_p * len(maps))(*[m.h for m in maps])
         check(abi.load().tsdf_integrate_batch(self.h, mp, len(maps), arr, len(scans), C.c_void_p(points.ptr)))
 
-    def set_launch(self, traverse_block=256, traverse_grid_cap=65536, fold_grid=4096):
-        cfg = abi.LaunchConfig(traverse_block, traverse_grid_cap, fold_grid, 0)
+    def set_launch(self, traverse_block=256, traverse_grid_cap=65536, fold_grid=4096, rays_per_lane=1,
+                   traverse_queue=0):
+        """Launch shapes of the pass (tsdf_launch_config).  Passes captured in a batch graph keep the shapes of
+        their capture."""
+        cfg = abi.LaunchConfig(traverse_block, traverse_grid_cap, fold_grid, rays_per_lane, traverse_queue, 0)
         check(abi.load().tsdf_engine_set_launch(self.h, C.byref(cfg)))
 
     def get_launch(self) -> dict:
         cfg = abi.LaunchConfig()
         check(abi.load().tsdf_engine_get_launch(self.h, C.byref(cfg)))
-        return dict(traverse_block=cfg.traverse_block, traverse_grid_cap=cfg.traverse_grid_cap, fold_grid=cfg.fold_grid)
+        return dict(traverse_block=cfg.traverse_block, traverse_grid_cap=cfg.traverse_grid_cap, fold_grid=cfg.fold_grid,
+                    rays_per_lane=cfg.rays_per_lane, traverse_queue=cfg.traverse_queue)
 
     def timing(self, on: bool):
         check(abi.load().tsdf_timing_enable(self.h, 1 if on else 0))

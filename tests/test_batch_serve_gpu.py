@@ -307,13 +307,17 @@ def test_autotune_knobs_do_not_change_results():
     params = configs.DENSE_PARAMS
     cfgs = [dict(traverse_block=64, fold_grid=512, buckets_log2=13, max_chain=64),
             dict(traverse_block=256, fold_grid=4096, buckets_log2=16, max_chain=64),
-            dict(traverse_block=128, fold_grid=1024, buckets_log2=15, max_chain=16)]
+            dict(traverse_block=128, fold_grid=1024, buckets_log2=15, max_chain=16),
+            dict(traverse_block=256, rays_per_lane=2, fold_grid=4096, buckets_log2=16, max_chain=64),
+            dict(traverse_block=192, rays_per_lane=4, fold_grid=2048, buckets_log2=15, max_chain=64),
+            dict(traverse_block=256, rays_per_lane=1, traverse_queue=1, fold_grid=4096, buckets_log2=16, max_chain=64),
+            dict(traverse_block=128, rays_per_lane=4, traverse_queue=1, fold_grid=4096, buckets_log2=15, max_chain=16)]
     with Engine(heap_blocks=1 << 18, total_buckets=1 << 19, max_batch_points=1 << 18) as e:
         buf = e.buffer(pool.points.nbytes)
         buf.upload(pool.points)
         ref = None
         for cfg in cfgs:
-            e.set_launch(traverse_block=cfg["traverse_block"], fold_grid=cfg["fold_grid"])
+            e.set_launch(**autotune.launch_kwargs(cfg))
             maps = [e.map(params, buckets_log2=cfg["buckets_log2"], max_chain=cfg["max_chain"])
                     for _ in range(n_maps)]
             g = BatchGraph(e, maps, max_points=n_maps * 65536, max_scans=n_maps)

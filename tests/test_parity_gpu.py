@@ -91,3 +91,21 @@ def test_gc_recycles_blocks(eng, oracle_lib):
     free0 = eng.free_blocks()
     run_sequence(eng, oracle_lib, params, w, synth.scans(w, stop=30), check_every=5)
     assert eng.free_blocks() == free0
+
+
+@pytest.mark.parametrize("launch", [dict(rays_per_lane=4), dict(rays_per_lane=2, traverse_queue=1),
+                                    dict(traverse_block=64, rays_per_lane=3, traverse_queue=1)])
+def test_dense_parity_across_traversal_shapes(eng, oracle_lib, launch):
+    """The traversal's chunking (rays per lane, the persistent work-queue grid) changes which rays share an LDS
+    block table, and with 4 x 256 rays per table the table overflows into the global fallback paths; none of it
+    may change a key, a touched voxel or a value beyond the tolerance."""
+    w = configs.DENSE
+    eng.set_launch(**launch)
+    try:
+        got = eng.get_launch()
+        for k, v in launch.items():
+            assert got[k] == v
+        r = run_sequence(eng, oracle_lib, configs.DENSE_PARAMS, w, synth.scans(w, stop=2), publish=False, gc=False)
+    finally:
+        eng.set_launch()
+    print("dense", launch, r[-1])
