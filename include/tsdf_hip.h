@@ -357,6 +357,20 @@ typedef struct {
 int tsdf_cloud2_to_points(tsdf_engine* eng, const tsdf_cloud2_layout* layout, const void* cloud, int cloud_on_device,
                           float* points_dev, int64_t capacity, int64_t* n_points);
 
+/* ---- downstream consumer of the Tsdf message -------------------------------
+ * Replaces: the message's debug consumer, debug_viz.cpp's fetchTSDF (referenced
+ * by README.md:80-85, outside the reference tree), which draws every voxel of
+ * a tsdf_package_msgs/Tsdf as a marker: green for sdf >= 0, red for sdf < 0.
+ * voxels_msg: count Voxel.msg records {sdf, weight, x, y, z} (20 bytes each,
+ * as tsdf_publish_msg writes them), in device memory when msg_on_device is
+ * nonzero, else in host memory.  The records are split by the sign of sdf on
+ * the GPU, each side keeping the message's order: nonneg_xyz receives the
+ * positions of the sdf >= 0 voxels (-0.0 included, as the comparison has it),
+ * neg_xyz those of the sdf < 0 voxels, float[count][3] each at most (host
+ * pointers); *n_nonneg + *n_neg = count.  Synchronous. */
+int tsdf_msg_markers(tsdf_engine* eng, const float* voxels_msg, int64_t count, int msg_on_device, float* nonneg_xyz,
+                     float* neg_xyz, int64_t* n_nonneg, int64_t* n_neg);
+
 /* Query of the derived constants initGlobalVars computes (H:1296-1315):
  * out_f = {half_voxel, block_size, half_block, block_eps, voxel_eps},
  * out_i = {MAX_VOXELS_TRAVERSED_PER_LIDAR_POINT, max_voxel_blocks_traversed}. */

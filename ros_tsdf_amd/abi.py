@@ -137,6 +137,7 @@ SIGNATURES = {
     "tsdf_device_copy": [vp, vp, vp, C.c_int64],
     "tsdf_derived_params": [C.POINTER(Params), fp, i32p],
     "tsdf_cloud2_to_points": [vp, C.POINTER(Cloud2Layout), vp, C.c_int, vp, C.c_int64, i64p],
+    "tsdf_msg_markers": [vp, vp, C.c_int64, C.c_int, fp, fp, i64p, i64p],
 }
 SPECIAL_RESTYPE = {"tsdf_last_error": C.c_char_p, "tsdf_engine_stream": vp}
 SPECIAL_ARGS = {"tsdf_last_error": [], "tsdf_engine_stream": [vp]}

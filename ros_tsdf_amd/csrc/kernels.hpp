@@ -2044,6 +2044,58 @@ __global__ void __launch_bounds__(256) k_import(Dev D, int map_slot, const int32
   }
 }
 
+// ---- downstream consumer: markers of a Tsdf message --------------------------------------
+// A stable split of Voxel.msg records {sdf, weight, x, y, z} by the sign of
+// sdf.  k_msg_count: records of each tile of kMsgTile with sdf >= 0;
+// k_msg_split: the tile's offset is the sum of the counts of the tiles
+// before it (read by the workgroup itself: a message has a few hundred
+// tiles), each record's rank inside its tile comes from a workgroup scan
+// over the tile in message order, and the record's position goes to the
+// sdf >= 0 list at the rank, or to the sdf < 0 list at its index minus it.
+constexpr int kMsgTile = 4096;
+__global__ void __launch_bounds__(256) k_msg_count(const float* __restrict__ msg, int64_t n, uint32_t* tile_cnt) {
+  __shared__ uint32_t sh[kScanSh];
+  const int64_t base = (int64_t)blockIdx.x * kMsgTile;
+  uint32_t c = 0;
+  for (int k = threadIdx.x; k < kMsgTile; k += blockDim.x) {
+    const int64_t e = base + k;
+    if (e < n && msg[5 * e] >= 0.f) ++c;
+  }
+  uint32_t tot;
+  (void)block_scan_excl(c, sh, tot);
+  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tot;
+}
+__global__ void __launch_bounds__(256) k_msg_split(const float* __restrict__ msg, int64_t n, const uint32_t* tile_cnt,
+                                                   float* nonneg, float* neg) {
+  __shared__ uint32_t sh[kScanSh];
+  __shared__ unsigned long long carry_sh;
+  // the tile's offset
+  unsigned long long part = 0;
+  for (int t = threadIdx.x; t < (int)blockIdx.x; t += blockDim.x) part += tile_cnt[t];
+  part = wave_sum(part);
+  if (threadIdx.x == 0) carry_sh = 0;
+  __syncthreads();
+  if (lane_id() == 0) atomicAdd(&carry_sh, part);
+  __syncthreads();
+  unsigned long long g = carry_sh;  // records with sdf >= 0 before this chunk
+  const int64_t base = (int64_t)blockIdx.x * kMsgTile;
+  for (int k0 = 0; k0 < kMsgTile; k0 += blockDim.x) {
+    const int64_t e = base + k0 + threadIdx.x;
+    const bool in = e < n;
+    const bool pos = in && msg[5 * e] >= 0.f;
+    uint32_t tot;
+    const uint32_t r = block_scan_excl(pos ? 1u : 0u, sh, tot);
+    if (in) {
+      const unsigned long long before = g + r;  // sdf >= 0 records before e
+      float* dst = pos ? nonneg + 3 * before : neg + 3 * ((unsigned long long)e - before);
+      dst[0] = msg[5 * e + 2];
+      dst[1] = msg[5 * e + 3];
+      dst[2] = msg[5 * e + 4];
+    }
+    g += tot;
+  }
+}
+
 // ---- garbage collection -----------------------------------------------------------------
 // mode 0: remove blocks beyond the gc radius (H:836-868); mode 1: remove all
 // (map teardown).

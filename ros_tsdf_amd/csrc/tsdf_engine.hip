@@ -1501,6 +1501,53 @@ int tsdf_device_copy(tsdf_engine* E, void* dst, const void* src, int64_t bytes) 
   return TSDF_OK;
 }
 
+// ---- downstream consumer -------------------------------------------------------------------
+int tsdf_msg_markers(tsdf_engine* E, const float* msg, int64_t n, int msg_on_device, float* nonneg, float* neg,
+                     int64_t* n_nonneg, int64_t* n_neg) {
+  if (!E || !n_nonneg || !n_neg || n < 0 || (n > 0 && (!msg || !nonneg || !neg)))
+    return fail(TSDF_ERR_INVALID_ARG, "bad argument");
+  if (n > ((int64_t)1 << 31) - kMsgTile) return fail(TSDF_ERR_CAPACITY, "a message holds at most 2^31 voxels here");
+  *n_nonneg = *n_neg = 0;
+  if (n == 0) return TSDF_OK;
+  CHK(set_device(E));
+  hipStream_t st = E->stream;
+  const int tiles = (int)((n + kMsgTile - 1) / kMsgTile);
+  float* dmsg = nullptr;
+  float* dout = nullptr;
+  uint32_t* dcnt = nullptr;
+  auto release = [&](int r) {
+    (void)hipStreamSynchronize(st);
+    if (!msg_on_device) (void)hipFree(dmsg);
+    (void)hipFree(dout);
+    (void)hipFree(dcnt);
+    return r;
+  };
+  int r;
+  if (!msg_on_device && (r = dev_alloc(&dmsg, 5 * n))) return release(r);
+  if ((r = dev_alloc(&dout, 3 * n)) || (r = dev_alloc(&dcnt, tiles))) return release(r);
+  const float* src = msg_on_device ? msg : dmsg;
+  if (!msg_on_device && hipMemcpyAsync(dmsg, msg, sizeof(float) * 5 * n, hipMemcpyHostToDevice, st) != hipSuccess)
+    return release(fail(TSDF_ERR_HIP, "H2D copy of the message failed"));
+  // the two lists share one device buffer, the sdf >= 0 list first: the
+  // sdf < 0 list starts where the count puts the end of the first
+  hipLaunchKernelGGL(k_msg_count, dim3(tiles), dim3(256), 0, st, src, n, dcnt);
+  std::vector<uint32_t> cnt(tiles);
+  if (hipMemcpyAsync(cnt.data(), dcnt, sizeof(uint32_t) * tiles, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return release(fail(TSDF_ERR_HIP, "marker count failed"));
+  int64_t np = 0;
+  for (uint32_t c : cnt) np += c;
+  hipLaunchKernelGGL(k_msg_split, dim3(tiles), dim3(256), 0, st, src, n, dcnt, dout, dout + 3 * np);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(nonneg, dout, sizeof(float) * 3 * np, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(neg, dout + 3 * np, sizeof(float) * 3 * (n - np), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return release(fail(TSDF_ERR_HIP, "marker split failed"));
+  *n_nonneg = np;
+  *n_neg = n - np;
+  return release(TSDF_OK);
+}
+
 // ---- input path --------------------------------------------------------------------------
 int tsdf_cloud2_to_points(tsdf_engine* E, const tsdf_cloud2_layout* L, const void* cloud, int cloud_on_device,
                           float* points_dev, int64_t capacity, int64_t* n_points) {

@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <mutex>
 
+#include "tsdf/tsdf_debug_viz.hpp"
 #include "tsdf/tsdf_handler.hpp"
 #include "tsdf_hip.h"
 
@@ -190,6 +191,29 @@ void TSDFHandler::garbageCollectDistantBlocks(Vector3f* lidar_position_d, HashTa
   float s[3];
   if (!m || !read_device_vec3(lidar_position_d, s) || tsdf_gc(m, s, nullptr) != TSDF_OK)
     report("garbageCollectDistantBlocks");
+}
+
+bool TsdfDebugViz::fetchTSDF(const VoxelMsg* voxels, uint32_t size, float voxel_size, double stamp_s) {
+  if (rendered_ && stamp_s - last_s_ < period_s_) return false;
+  tsdf_engine* E = engine();
+  if (!E) return false;
+  std::vector<Vector3f> g(size), r(size);
+  int64_t ng = 0, nr = 0;
+  if (size > 0 &&
+      tsdf_msg_markers(E, reinterpret_cast<const float*>(voxels), size, 0, reinterpret_cast<float*>(g.data()),
+                       reinterpret_cast<float*>(r.data()), &ng, &nr) != TSDF_OK) {
+    report("fetchTSDF");
+    return false;
+  }
+  g.resize(ng);
+  r.resize(nr);
+  green_.swap(g);
+  red_.swap(r);
+  scale_ = voxel_size;
+  last_s_ = stamp_s;
+  rendered_ = true;
+  ++renders_;
+  return true;
 }
 
 }  // namespace tsdf

@@ -214,6 +214,24 @@ class Engine:
                                                C.c_void_p(out.ptr), out.nbytes // 16, C.byref(n)))
         return int(n.value)
 
+    def msg_markers(self, msg):
+        """The Tsdf message's debug markers (the consumer of README.md:80-85, debug_viz.cpp's fetchTSDF): msg is
+        a float32 [n, 5] array of Voxel.msg records {sdf, weight, x, y, z} on the host, or a DeviceBuffer of n x 20
+        bytes.  Returns (positions of the sdf >= 0 voxels, positions of the sdf < 0 voxels), float32 [k, 3] each,
+        in message order."""
+        on_dev = isinstance(msg, DeviceBuffer)
+        if on_dev:
+            ptr, n = msg.ptr, msg.nbytes // 20
+        else:
+            msg = np.ascontiguousarray(msg, dtype=np.float32).reshape(-1, 5)
+            ptr, n = msg.ctypes.data, msg.shape[0]
+        nonneg = np.empty((max(n, 1), 3), np.float32)
+        neg = np.empty((max(n, 1), 3), np.float32)
+        a, b = C.c_int64(), C.c_int64()
+        check(abi.load().tsdf_msg_markers(self.h, C.c_void_p(ptr), n, 1 if on_dev else 0, _fptr(nonneg), _fptr(neg),
+                                          C.byref(a), C.byref(b)))
+        return nonneg[:a.value], neg[:b.value]
+
 
 class Map:
     def __init__(self, engine: Engine, params, buckets_log2=16, overflow_buckets=0, max_chain=64, block_quota=0,

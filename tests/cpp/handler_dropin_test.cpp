@@ -9,10 +9,14 @@
 //            -> garbageCollectDistantBlocks
 //   stages:  getVoxelBlocks -> allocateVoxelBlocks -> getVoxelsAndUpdate ->
 //            retrievePublishableVoxels -> garbageCollectDistantBlocks
+//   viz:     scan, then the published voxels as a Tsdf message (Voxel.msg
+//            records, as the reference's Publisher builds it) handed to the
+//            debug consumer tsdf::TsdfDebugViz, stamped 2 s apart
 //
-//   handler_dropin_test <in.bin> <out.bin> [scan|members|stages]
+//   handler_dropin_test <in.bin> <out.bin> [scan|members|stages|viz]
 // in.bin : int32 nscans, float params[5]; per scan: int32 n, float sensor[3], float points[n][4]
-// out.bin: per scan: int32 count, then count x {float x, y, z, sdf, weight}
+// out.bin: per scan: int32 count, then count x {float x, y, z, sdf, weight}; in viz mode then int32 rendered,
+//          int32 n_green, int32 n_red, n_green x float[3], n_red x float[3] (the markers after the callback)
 #include <hip/hip_runtime_api.h>
 
 #include <cstdio>
@@ -21,6 +25,7 @@
 #include <memory>
 #include <vector>
 
+#include "tsdf/tsdf_debug_viz.hpp"
 #include "tsdf/tsdf_handler.hpp"
 
 namespace {
@@ -49,10 +54,12 @@ struct DeviceScan {
 
 int main(int argc, char** argv) {
   if (argc != 3 && argc != 4) {
-    std::fprintf(stderr, "usage: %s in.bin out.bin [scan|members|stages]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s in.bin out.bin [scan|members|stages|viz]\n", argv[0]);
     return 2;
   }
   const int mode = argc == 4 ? (!std::strcmp(argv[3], "members") ? 1 : !std::strcmp(argv[3], "stages") ? 2 : 0) : 0;
+  const bool viz_mode = argc == 4 && !std::strcmp(argv[3], "viz");
+  tsdf::TsdfDebugViz viz;
   FILE* in = std::fopen(argv[1], "rb");
   FILE* out = std::fopen(argv[2], "wb");
   if (!in || !out) return 2;
@@ -96,6 +103,17 @@ int main(int argc, char** argv) {
     for (int i = 0; i < size; ++i) {
       const float rec[5] = {pos[i](0), pos[i](1), pos[i](2), data[i].sdf, data[i].weight};
       std::fwrite(rec, 4, 5, out);
+    }
+    if (viz_mode) {
+      std::vector<tsdf::VoxelMsg> msg(size);
+      for (int i = 0; i < size; ++i) msg[i] = {data[i].sdf, data[i].weight, pos[i](0), pos[i](1), pos[i](2)};
+      const int rendered = viz.fetchTSDF(msg.data(), (uint32_t)size, params[0], 2.0 * k) ? 1 : 0;
+      const int ng = (int)viz.green().size(), nr = (int)viz.red().size();
+      std::fwrite(&rendered, 4, 1, out);
+      std::fwrite(&ng, 4, 1, out);
+      std::fwrite(&nr, 4, 1, out);
+      if (ng) std::fwrite(viz.green().data(), 12, ng, out);
+      if (nr) std::fwrite(viz.red().data(), 12, nr, out);
     }
   }
   delete handler;

@@ -54,3 +54,52 @@ def test_dropin_handler_matches_oracle(tmp_path, oracle_lib, mode):
                         configs.WEIGHT_TOL_REL * params[2])
     assert at == len(raw)
     om.close()
+
+
+def test_debug_viz_consumer(tmp_path):
+    """The downstream consumer (tsdf::TsdfDebugViz, the debug node of README.md:80-85) fed each scan's Tsdf
+    message 2 s apart: it renders the first message and then one whenever 5 s have passed (scans 0, 3, 6, 9),
+    and a render's green/red markers are exactly the message's sdf >= 0 / sdf < 0 voxel positions, in message
+    order; between renders the markers stay those of the last render."""
+    if not os.path.exists(_build.HANDLER_TEST_BIN):
+        pytest.fail("drop-in test program %s is not built" % _build.HANDLER_TEST_BIN)
+    params = configs.SMALL_PARAMS
+    scans = [(p, s) for _, p, s in synth.scans(configs.SMALL, stop=10)]
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as f:
+        f.write(struct.pack("<i5f", len(scans), *params))
+        for p, s in scans:
+            f.write(struct.pack("<i3f", p.shape[0], *[float(x) for x in s]))
+            f.write(np.ascontiguousarray(p, np.float32).tobytes())
+    out = tmp_path / "out.bin"
+    env = dict(os.environ, TSDF_HEAP_BLOCKS="65536")
+    r = subprocess.run([_build.HANDLER_TEST_BIN, str(inp), str(out), "viz"], env=env, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = out.read_bytes()
+    at = 0
+    last = None
+    renders = []
+    for k in range(len(scans)):
+        (n,) = struct.unpack_from("<i", raw, at)
+        at += 4
+        rec = np.frombuffer(raw, np.float32, count=5 * n, offset=at).reshape(n, 5)
+        at += 20 * n
+        rendered, ng, nr = struct.unpack_from("<3i", raw, at)
+        at += 12
+        green = np.frombuffer(raw, np.float32, count=3 * ng, offset=at).reshape(ng, 3)
+        at += 12 * ng
+        red = np.frombuffer(raw, np.float32, count=3 * nr, offset=at).reshape(nr, 3)
+        at += 12 * nr
+        if rendered:
+            renders.append(k)
+            assert n > 0
+            pos = rec[:, 3] >= 0
+            assert np.array_equal(green.view(np.uint32), rec[pos][:, :3].view(np.uint32))
+            assert np.array_equal(red.view(np.uint32), rec[~pos][:, :3].view(np.uint32))
+            assert ng > 0 and nr > 0
+            last = (green.copy(), red.copy())
+        else:
+            assert np.array_equal(green, last[0]) and np.array_equal(red, last[1])
+    assert renders == [0, 3, 6, 9]
+    assert at == len(raw)
