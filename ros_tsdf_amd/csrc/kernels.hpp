@@ -125,8 +125,9 @@ __device__ __forceinline__ void write_stat_row(const Dev& D, int region, const u
 struct StatGrids {
   int32_t n[SR_COUNT];
 };
-__global__ void __launch_bounds__(256) k_pass_stats(Dev D, StatGrids g, uint32_t mask) {
-  __shared__ unsigned long long sh[4];
+constexpr int kPassStatsThreads = 1024;
+__global__ void __launch_bounds__(kPassStatsThreads) k_pass_stats(Dev D, StatGrids g, uint32_t mask) {
+  __shared__ unsigned long long sh[kPassStatsThreads / 64];
   const int st = blockIdx.x;
   if (st >= ST_COUNT || !(mask & (1u << st))) return;
   // (region, column) pairs that hold the statistic
@@ -150,12 +151,28 @@ __global__ void __launch_bounds__(256) k_pass_stats(Dev D, StatGrids g, uint32_t
   for (int q = 0; q < 2; ++q) {
     if (reg[q] < 0) continue;
     const uint32_t* r = D.srows + (int64_t)stat_region_base(reg[q]) * kStatCols + col[q];
-    for (int x = threadIdx.x; x < g.n[reg[q]]; x += blockDim.x) v += r[(int64_t)x * kStatCols];
+    const int n = g.n[reg[q]];
+    // four rows per thread per step, loaded before they are added (a
+    // 65536-row region is 16 steps of 1024 threads)
+    for (int x0 = threadIdx.x; x0 < n; x0 += 4 * kPassStatsThreads) {
+      uint32_t t[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int x = x0 + u * kPassStatsThreads;
+        t[u] = x < n ? r[(int64_t)x * kStatCols] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v += t[u];
+    }
   }
   v = wave_sum(v);
   if (lane_id() == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) D.ctr->stats[st] = sh[0] + sh[1] + sh[2] + sh[3];
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (int w = 0; w < kPassStatsThreads / 64; ++w) tot += sh[w];
+    D.ctr->stats[st] = tot;
+  }
 }
 
 // Zeroes the partial slots of the statistics selected by mask.

@@ -1193,7 +1193,7 @@ int tsdf_process_scan(tsdf_map* m, const float* points_host, int64_t n, const fl
   if (!r) {
     // the pass's failure counters, reduced while the stream is still in order
     // (publish and gc keep their own statistics)
-    hipLaunchKernelGGL(k_pass_stats, dim3(ST_COUNT), dim3(256), 0, E->stream, E->view(), E->last_grids, kDropStats);
+    hipLaunchKernelGGL(k_pass_stats, dim3(ST_COUNT), dim3(kPassStatsThreads), 0, E->stream, E->view(), E->last_grids, kDropStats);
     e = hipGetLastError();
     if (e != hipSuccess) r = fail(TSDF_ERR_HIP, "statistics reduction failed: %s", hipGetErrorString(e));
   }
@@ -1216,7 +1216,7 @@ int tsdf_last_stats(tsdf_engine* E, tsdf_scan_stats* out) {
   CHK(set_device(E));
   // the last pass's statistics stay in their workgroup rows until the next
   // pass overwrites them: sum them now
-  hipLaunchKernelGGL(k_pass_stats, dim3(ST_COUNT), dim3(256), 0, E->stream, E->view(), E->last_grids, kPassStats);
+  hipLaunchKernelGGL(k_pass_stats, dim3(ST_COUNT), dim3(kPassStatsThreads), 0, E->stream, E->view(), E->last_grids, kPassStats);
   HIPCHK(hipGetLastError());
   CHK(read_ctr(E));
   const unsigned long long* s = E->ctr_host->stats;
