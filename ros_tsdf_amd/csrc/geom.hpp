@@ -172,6 +172,34 @@ struct CellWalk {
     }
     return true;
   }
+
+  // advance() with no early return, for loops that end at the first false:
+  // the step is applied whatever the guard says (a walk the guard ends is not
+  // used again), so the loop body has a single exit and no nested branches.
+  __device__ __forceinline__ bool advance_always(bool xing[3]) {
+    const bool gt01 = tMax[0] > tMax[1], lt01 = tMax[0] < tMax[1];
+    const bool gt02 = tMax[0] > tMax[2];
+    const bool gt12 = tMax[1] > tMax[2];
+    const bool sx = !gt01 && !gt02;
+    const bool sy = (gt01 && !gt12) || (!lt01 && !gt01 && !gt02);
+    const bool sz = !((gt01 ? tMax[1] : tMax[0]) < tMax[2]);
+    const bool st[3] = {sx, sy, sz};
+    int32_t kn[3], d[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      cur[i] = st[i] ? cur[i] + step[i] : cur[i];
+      tMax[i] = st[i] ? tMax[i] + tDelta[i] : tMax[i];
+      kn[i] = floor_int(cur[i] * scale);
+      d[i] = kn[i] - k[i];
+    }
+    const uint32_t far = max(max((uint32_t)(d[0] + 1), (uint32_t)(d[1] + 1)), (uint32_t)(d[2] + 1));
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      xing[i] = (uint32_t)(kn[i] ^ k[i]) > 7u;
+      k[i] = kn[i];
+    }
+    return (d[0] | d[1] | d[2]) != 0 && far <= 2u;
+  }
 };
 
 // The cells of traverseVolume, in order: at most max_cells are emitted and
@@ -183,23 +211,21 @@ __device__ __forceinline__ int walk_cells(const float a[3], const float b[3], fl
   trunc = false;
   CellWalk w(a, b, L, scale, iscale, half);
   int n = 0;
-  while (!w.at_end()) {
+  // each turn emits the current cell (the end cell included) and steps on;
+  // the step is computed whether or not the walk goes on
+  for (;;) {
     if (n == max_cells) {
       trunc = true;
-      return n;
+      break;
     }
+    const bool last = w.at_end();
     const bool go = emit(w.k[0], w.k[1], w.k[2]);
     ++n;
-    if (!go) return n;
     bool x[3];
-    if (!w.advance(x)) return n;
+    const bool moved = w.advance_always(x);
+    if (last || !go || !moved) break;
   }
-  if (n == max_cells) {
-    trunc = true;
-    return n;
-  }
-  emit(w.k[0], w.k[1], w.k[2]);
-  return n + 1;
+  return n;
 }
 
 // The same walk (voxel cells, blocks of 8^3) summarised as runs: every axis
@@ -237,20 +263,23 @@ __device__ __forceinline__ int walk_runs(const float a[3], const float b[3], flo
   };
   if (max_cells <= 32) {
     // every cell fits one window: a loop with no exit but the walk's own end
+    // Each turn computes the next step whatever happens and has one exit: the
+    // cell limit (the end cell too: walk_cells cuts it as well), the end cell,
+    // or the precision guard.
     uint32_t cm[3] = {0u, 0u, 0u};
     for (;;) {
-      if (n == max_cells) {  // (the end cell too: walk_cells cuts it as well)
-        trunc = true;
-        break;
-      }
+      const bool cut = n == max_cells;
       const bool last = w.at_end();
-      ++n;
       bool x[3];
-      if (last || !w.advance(x)) break;
+      const bool moved = w.advance_always(x);
+      trunc = trunc || cut;
+      n += cut ? 0 : 1;
+      const bool go = !cut && !last && moved;
       // now at cell n; a cell past max_cells (n = 32) is never counted
-      const uint32_t bit = n < 32 ? 1u << n : 0u;
+      const uint32_t bit = go && n < 32 ? 1u << n : 0u;
 #pragma unroll
-      for (int i = 0; i < 3; ++i) cm[i] = x[i] ? cm[i] | bit : cm[i];
+      for (int i = 0; i < 3; ++i) cm[i] |= x[i] ? bit : 0u;
+      if (!go) break;
     }
     drain(cm, n);
     if (open > 0) on_run(blk[0], blk[1], blk[2], open);
