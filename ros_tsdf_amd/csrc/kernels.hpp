@@ -415,6 +415,7 @@ __device__ __forceinline__ int assign_heap(const Dev& D, bool active, uint32_t e
 // LDS table falls back to per-run global atomics.
 constexpr int kBinSlots = 1024;
 constexpr int kBlkReg = 4;  // block-walk cells a lane keeps in registers (MAX_BLK of the configs)
+constexpr int kPairLds = 4;  // runs per ray whose pairs k_bin keeps in LDS
 constexpr uint32_t kPairFail = 0xFFFFFFFFu;
 constexpr uint32_t kPairDirect = 0x80000000u;
 
@@ -513,6 +514,10 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched 
   __shared__ uint32_t ltouch[kBinSlots];
   __shared__ uint8_t ltouch_map[kBinSlots];
   __shared__ uint16_t locc[kBinSlots];
+  // one ray per lane: the (entry or slot, record offset) pairs of a ray's
+  // first kPairLds runs, kept in LDS from the voxel walk to phase C; later
+  // runs go through the pair rows in global memory
+  __shared__ uint2 lpair[kMulti ? 1 : kPairLds][kMulti ? 1 : 256];
   __shared__ int n_new, n_touch, g_touch, g_new, n_occ, next_chunk;
   const PassHdr hdr = inl.nscans >= 0 ? inl.hdr : *D.hdr;
   const int P = hdr.total;
@@ -702,18 +707,25 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched 
             if (bsl[u] >= 0 && bk0[u] == kb[0] && bk1[u] == kb[1] && bk2[u] == kb[2]) slot = bsl[u];
           bool added = false;
           if (slot < 0) slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb), &added);
+          auto put = [&](uint32_t pe, uint32_t pr) {
+            if (!kMulti && nj < kPairLds) {
+              lpair[kMulti ? 0 : nj][threadIdx.x] = make_uint2(pe, pr);
+            } else {
+              D.pair_e[(int64_t)nj * stride + i] = pe;
+              D.pair_r[(int64_t)nj * stride + i] = pr;
+            }
+          };
           if (slot >= 0) {
             if (added) locc[atomicAdd(&n_occ, 1)] = (uint16_t)slot;
             const uint32_t local = atomicAdd(&trec[slot], (uint32_t)rn);
-            D.pair_e[(int64_t)nj * stride + i] = (uint32_t)slot;
-            D.pair_r[(int64_t)nj * stride + i] = local | ((uint32_t)rn << 24);
+            put((uint32_t)slot, local | ((uint32_t)rn << 24));
           } else {
             bool ins;
             const int e = find_or_insert(D, m, pack_key(kb[0], kb[1], kb[2]), ins);
             if (e < 0) {
               st[6]++;
               st[7] += rn;
-              D.pair_e[(int64_t)nj * stride + i] = kPairFail;
+              put(kPairFail, 0u);
             } else {
               if (ins) note_new(e, m.slot);
               const unsigned long long old = atomicAdd(&D.blk_cnt[e], (unsigned long long)rn << 32);
@@ -722,8 +734,7 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched 
                 D.touched[t] = (uint32_t)e;
                 D.touched_map[t] = (uint8_t)m.slot;
               }
-              D.pair_e[(int64_t)nj * stride + i] = kPairDirect | (uint32_t)e;
-              D.pair_r[(int64_t)nj * stride + i] = (uint32_t)(old >> 32);
+              put(kPairDirect | (uint32_t)e, (uint32_t)(old >> 32));
               st[3] += rn;
             }
           }
@@ -754,19 +765,14 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched 
     }
     __syncthreads();
     const int nn = min(n_new, kBinSlots);
+    // The workgroup's places in the pass's touched and new-entry lists: the
+    // atomics are issued now and their results read after phase C, which
+    // does not need them, so their round trip overlaps it.
+    int gt = 0, gn = 0, nc = 0;
     if (threadIdx.x == 0) {
-      g_touch = n_touch ? atomicAdd(&D.ctr->touched_n, n_touch) : 0;
-      g_new = nn ? atomicAdd(&D.ctr->new_n, nn) : 0;
-      next_chunk = bs.queue ? (int)gridDim.x + atomicAdd(&D.ctr->bin_next, 1) : c + (int)gridDim.x;
-    }
-    __syncthreads();
-    for (int x = threadIdx.x; x < n_touch; x += blockDim.x) {
-      D.touched[g_touch + x] = ltouch[x];
-      D.touched_map[g_touch + x] = ltouch_map[x];
-    }
-    for (int x = threadIdx.x; x < nn; x += blockDim.x) {
-      D.new_list[g_new + x] = lnew[x];
-      D.new_map[g_new + x] = lnew_map[x];
+      gt = n_touch ? atomicAdd(&D.ctr->touched_n, n_touch) : 0;
+      gn = nn ? atomicAdd(&D.ctr->new_n, nn) : 0;
+      nc = bs.queue ? (int)gridDim.x + atomicAdd(&D.ctr->bin_next, 1) : c + (int)gridDim.x;
     }
     // ---- phase C: final (entry, record offset) of every run ----
     for (int r = 0; r < R; ++r) {
@@ -777,23 +783,48 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched 
         if (u == r) nj = njr[u];
       if (i >= P) continue;
       for (int j = 0; j < nj; ++j) {
-        const uint32_t v = D.pair_e[(int64_t)j * stride + i];
-        if (v == kPairFail || (v & kPairDirect)) {
-          if (v != kPairFail) D.pair_e[(int64_t)j * stride + i] = v & ~kPairDirect;
-          continue;
-        }
-        const uint32_t pr = D.pair_r[(int64_t)j * stride + i];
-        const uint32_t rn = pr >> 24;
-        const int e = tent[v];
-        if (e < 0) {
-          D.pair_e[(int64_t)j * stride + i] = kPairFail;
-          st[7] += rn;
+        const bool inl = !kMulti && j < kPairLds;
+        uint32_t v, pr;
+        if (inl) {
+          const uint2 q = lpair[kMulti ? 0 : j][threadIdx.x];
+          v = q.x;
+          pr = q.y;
         } else {
-          D.pair_e[(int64_t)j * stride + i] = (uint32_t)e;
-          D.pair_r[(int64_t)j * stride + i] = tbase[v] + (pr & 0xFFFFFFu);
-          st[3] += rn;
+          v = D.pair_e[(int64_t)j * stride + i];
+          pr = D.pair_r[(int64_t)j * stride + i];
         }
+        uint32_t fe, fr = pr;
+        if (v == kPairFail || (v & kPairDirect)) {
+          fe = v == kPairFail ? kPairFail : v & ~kPairDirect;
+        } else {
+          const uint32_t rn = pr >> 24;
+          const int e = tent[v];
+          if (e < 0) {
+            fe = kPairFail;
+            st[7] += rn;
+          } else {
+            fe = (uint32_t)e;
+            fr = tbase[v] + (pr & 0xFFFFFFu);
+            st[3] += rn;
+          }
+        }
+        D.pair_e[(int64_t)j * stride + i] = fe;
+        if (fe != kPairFail) D.pair_r[(int64_t)j * stride + i] = fr;
       }
+    }
+    if (threadIdx.x == 0) {
+      g_touch = gt;
+      g_new = gn;
+      next_chunk = nc;
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < n_touch; x += blockDim.x) {
+      D.touched[g_touch + x] = ltouch[x];
+      D.touched_map[g_touch + x] = ltouch_map[x];
+    }
+    for (int x = threadIdx.x; x < nn; x += blockDim.x) {
+      D.new_list[g_new + x] = lnew[x];
+      D.new_map[g_new + x] = lnew_map[x];
     }
     c = next_chunk;
     __syncthreads();
