@@ -637,7 +637,9 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched 
           bool tb = false;
           const int nb = walk_cells(a, b, m.BL, m.sc_b, m.isc_b, m.half_BL, m.max_blk, tb, [&](int32_t k0, int32_t k1, int32_t k2) {
             const int32_t kb[3] = {k0, k1, k2};
-            const int slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb));
+            bool added = false;
+            const int slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb), &added);
+            if (added) locc[atomicAdd(&n_occ, 1)] = (uint16_t)slot;  // the table's list
             if (slot < 0) {
               bool ins;
               const int e = find_or_insert(D, m, pack_key(k0, k1, k2), ins);
@@ -661,15 +663,8 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched 
       }
     }
     __syncthreads();
-    // ---- the table's blocks listed; their home buckets are read now and
-    // used after the voxel walks, which hide the memory latency ----
-    for (int x0 = 0; x0 < kBinSlots; x0 += blockDim.x) {
-      const int x = x0 + threadIdx.x;
-      const bool occ = x < kBinSlots && tkey[x] != kEmptyKey;
-      const int at = wave_append(&n_occ, occ);
-      if (occ) locc[at] = (uint16_t)x;
-    }
-    __syncthreads();
+    // ---- the blocks the block walks listed: their home buckets are read now
+    // and used after the voxel walks, which hide the memory latency ----
     const int n_listed = n_occ;
     // (one bucket per thread: a second one held through the walks would cost
     // the registers of the third wave per SIMD)
