@@ -1436,9 +1436,9 @@ __device__ __forceinline__ void bitonic_sort_lds(unsigned long long* a, int n) {
 // updates; each record is then placed by its rank (by point) inside its
 // voxel, and each voxel's thread folds its updates in point order.  rin/rout
 // hold TILE records.
-template <class State, int TILE, int NT>
+template <class State, int TILE, int NT, class StartT = uint32_t>
 __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, unsigned long long* rin,
-                                                 unsigned long long* rout, uint32_t* cnt, uint32_t* start,
+                                                 unsigned long long* rout, uint32_t* cnt, StartT* start,
                                                  uint32_t* sh, unsigned long long& uniq) {
   static_assert(NT == 256 || NT == 512 || NT == 1024, "workgroup of 256, 512 or 1024 threads");
   constexpr int kPer = TILE / NT;                                         // records per thread
@@ -1800,11 +1800,11 @@ __device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long
 // grid larger than what fits on the chip at once, the dispatcher starts a
 // new workgroup whenever one finishes, which balances the uneven items.
 template <class State>
-__global__ void __launch_bounds__(256, 7) k_fold_a(Dev D) {
+__global__ void __launch_bounds__(256, 8) k_fold_a(Dev D) {
   __shared__ unsigned long long rin[kCountTileS];
   __shared__ unsigned long long rout[kCountTileS];
   __shared__ uint32_t cnt[kVoxelsPerBlock];
-  __shared__ uint32_t start[kVoxelsPerBlock + 1];
+  __shared__ uint16_t start[kVoxelsPerBlock + 1];  // (offsets within a tile of kCountTileS records)
   __shared__ uint32_t sh[kScanSh];
   unsigned long long uniq = 0;
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG]);
@@ -1824,7 +1824,7 @@ __global__ void __launch_bounds__(256, 7) k_fold_a(Dev D) {
   static_assert(kCountTileS >= 4 * kWgSplit, "a wave's strip of kWgSplit records in rin");
   for (int it = blockIdx.x; it < total; it += gridDim.x) {
     if (it < n_wg) {
-      fold_count_block<State, kCountTileS, 256>(D, dwg[it], rin, rout, cnt, start, sh, uniq);
+      fold_count_block<State, kCountTileS, 256, uint16_t>(D, dwg[it], rin, rout, cnt, start, sh, uniq);
     } else if (it < n_wg + cl) {
       const int q = (it - n_wg) * kl + wave;
       if (q < n_wgl) {
