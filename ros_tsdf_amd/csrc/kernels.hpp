@@ -967,6 +967,7 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int32_t* n_ptr, Count 
 __global__ void __launch_bounds__(256) k_pass_scan(Dev D) {
   __shared__ uint32_t sh[kScanSh];
   const int n = ld_acq(&D.ctr->touched_n);
+  const int nn = ld_acq(&D.ctr->new_n);  // (read with the other count: one round trip)
   const int ntiles = (n + kScanTile - 1) / kScanTile;
   const CountTouchedRecords cnt{D.touched, D.blk_cnt};
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -983,7 +984,6 @@ __global__ void __launch_bounds__(256) k_pass_scan(Dev D) {
   // heap failures include quota failures, which are also counted apart
   // (columns of the SR_SCAN rows: ST_NEW_BLOCKS, ST_HEAP_FAIL, ST_QUOTA_FAIL)
   uint32_t st[3] = {0, 0, 0};
-  const int nn = ld_acq(&D.ctr->new_n);
   const int nchunks = (nn + 255) / 256;
   for (int c = gridDim.x - 1 - blockIdx.x; c < nchunks; c += gridDim.x) {
     const int x = c * 256 + threadIdx.x;
@@ -1205,6 +1205,7 @@ __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
   __shared__ unsigned long long sh64[kScanSh];
   __shared__ int lcnt[FC_COUNT], gbase[FC_COUNT];
   const int n = ld_acq(&D.ctr->touched_n);
+  const int nf = ld_acq(&D.ctr->fail_n);  // (read with the other count: one round trip)
   const int ntiles = (n + kScanTile - 1) / kScanTile;
   unsigned long long dropped = 0;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -1267,7 +1268,6 @@ __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
       if (cls[k] >= 0) D.fold_desc[D.fold_base[cls[k]] + gbase[cls[k]] + rank[k]] = dsc[k];
     __syncthreads();
   }
-  const int nf = ld_acq(&D.ctr->fail_n);
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nf; q += gridDim.x * blockDim.x)
     D.keys[D.fail_list[q]] = kEmptyKey;
   const uint32_t row[1] = {(uint32_t)dropped};
