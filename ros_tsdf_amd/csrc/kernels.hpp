@@ -844,6 +844,8 @@ struct CountPlain {
 // scan, then a scan of the wave totals.  sh holds kScanSh entries: the
 // waves' offsets, then the total.
 constexpr int kScanSh = 17;
+// NW: the workgroup's waves when known at compile time (0: from blockDim).
+template <int NW = 0>
 __device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* sh, uint32_t& total) {
   const int lane = lane_id();
   const int w = threadIdx.x >> 6;
@@ -857,7 +859,10 @@ __device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* sh, ui
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t acc = 0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+    const int nw = NW > 0 ? NW : (int)(blockDim.x >> 6);
+#pragma unroll
+    for (int k = 0; k < (NW > 0 ? NW : 16); ++k) {
+      if (NW == 0 && k >= nw) break;
       const uint32_t t = sh[k];
       sh[k] = acc;
       acc += t;
@@ -1491,7 +1496,7 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
       sum += cv[u];
     }
     uint32_t tot;
-    uint32_t run = block_scan_excl(sum, sh, tot);
+    uint32_t run = block_scan_excl<NT / 64>(sum, sh, tot);
     if (own) {
 #pragma unroll
       for (int u = 0; u < kVpt; ++u) {
@@ -1585,7 +1590,7 @@ __device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, un
   if (mine > (uint32_t)kRangeSpan) too_big = 1;
   if (mine != 0) atomicMax(&maxc_sh, (int)mine);
   uint32_t tot;
-  const uint32_t ex = block_scan_excl(mine, sh, tot);  // (its barriers publish too_big and maxc_sh)
+  const uint32_t ex = block_scan_excl<NT / 64>(mine, sh, tot);  // (its barriers publish too_big and maxc_sh)
   if (too_big) return false;
   const uint32_t span = (uint32_t)(kCountTile - maxc_sh);  // >= kRangeSpan
   // voxel t: its start, range, state.  The ranges are those of the voxels
@@ -1712,7 +1717,7 @@ __device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long
       sum += v[u];
     }
     uint32_t tot;
-    uint32_t run = block_scan_excl(sum, sh, tot);
+    uint32_t run = block_scan_excl<NT / 64>(sum, sh, tot);
     if (tot == 0) break;  // (tot is the same for every thread)
     if (t == 0) {
       bin_sh = -1;
@@ -1745,7 +1750,7 @@ __device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long
       __syncthreads();
       const uint32_t w = hist[t];
       uint32_t tot2;
-      const uint32_t e = block_scan_excl(w, sh, tot2) + (uint32_t)base_sh;
+      const uint32_t e = block_scan_excl<NT / 64>(w, sh, tot2) + (uint32_t)base_sh;
       if (e <= (uint32_t)kCountTile && e + w > (uint32_t)kCountTile) hi_sh = b0 + t;
       __syncthreads();
     }
@@ -1769,7 +1774,7 @@ __device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long
     {
       const uint32_t cv = t < kVoxelsPerBlock ? cnt[t] : 0u;
       uint32_t tot3;
-      const uint32_t ex = block_scan_excl(cv, sh, tot3);  // (its barriers also order the padding)
+      const uint32_t ex = block_scan_excl<NT / 64>(cv, sh, tot3);  // (its barriers also order the padding)
       if (t < kVoxelsPerBlock) start[t] = ex;
       if (t == 0) start[kVoxelsPerBlock] = tot3;
     }
