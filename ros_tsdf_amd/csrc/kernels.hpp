@@ -1194,6 +1194,28 @@ __device__ __forceinline__ void weight_of(float d, float& w, float& wd) {
   wd = w * d;
 }
 
+// Folds the run rin[b..e) of one voxel's records into (S, W), in order.  The
+// distances are read from LDS four at a time and their weights computed
+// before the four serial fold steps, so the chain of steps does not wait for
+// an LDS read at every step.
+__device__ __forceinline__ void fold_run(float& S, float& W, const unsigned long long* rin, int b, int e,
+                                         float maxW) {
+  const uint32_t* lo = reinterpret_cast<const uint32_t*>(rin);  // (the distance is a record's low word)
+  int i = b;
+  for (; i + 4 <= e; i += 4) {
+    float w[4], wd[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) weight_of(__uint_as_float(lo[2 * (i + u)]), w[u], wd[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) fold_step(S, W, w[u], wd[u], maxW);
+  }
+  for (; i < e; ++i) {
+    float w, wd;
+    weight_of(__uint_as_float(lo[2 * i]), w, wd);
+    fold_step(S, W, w, wd, maxW);
+  }
+}
+
 __device__ __forceinline__ int rec_voxel(unsigned long long r) { return (int)(r >> 55); }
 __device__ __forceinline__ int rec_point(unsigned long long r) { return (int)((r >> 32) & ((1u << kPointIdxBits) - 1u)); }
 
@@ -1525,14 +1547,16 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
     const int b0 = (int)start[l], b1 = (int)start[l + 1];
     int rank = 0;
     int j = b0;
+    const uint32_t* hw = reinterpret_cast<const uint32_t*>(rout) + 1;
+    const uint32_t rh = (uint32_t)(r >> 32);
     for (; j + 8 <= b1; j += 8) {
-      unsigned long long v[8];
+      uint32_t v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = rout[j + u];
+      for (int u = 0; u < 8; ++u) v[u] = hw[2 * (j + u)];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) rank += v[u] < r ? 1 : 0;
+      for (int u = 0; u < 8; ++u) rank += v[u] < rh ? 1 : 0;
     }
-    for (; j < b1; ++j) rank += rout[j] < r ? 1 : 0;
+    for (; j < b1; ++j) rank += hw[2 * j] < rh ? 1 : 0;
     rin[b0 + rank] = r;
   }
   __syncthreads();
@@ -1540,12 +1564,7 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
   for (int u = 0; u < kVpt; ++u) {
     const int l = threadIdx.x + NT * u;
     if (l >= kVoxelsPerBlock) continue;
-    const int e = (int)start[l + 1];
-    for (int i = (int)start[l]; i < e; ++i) {
-      float w, wd;
-      weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
-      fold_step(S[u], W[u], w, wd, maxW);
-    }
+    fold_run(S[u], W[u], rin, (int)start[l], (int)start[l + 1], maxW);
     if (h[u]) {
       State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, S[u], W[u]);
       uniq++;
@@ -1628,14 +1647,16 @@ __device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, un
       const int b0 = (int)(start[l] - base), b1 = (int)(start[l + 1] - base);
       int rank = 0;
       int j = b0;
-      for (; j + 4 <= b1; j += 4) {
-        unsigned long long v[4];
+      const uint32_t* hw = reinterpret_cast<const uint32_t*>(rout) + 1;
+      const uint32_t rh = (uint32_t)(rec >> 32);
+      for (; j + 8 <= b1; j += 8) {
+        uint32_t v[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = rout[j + u];
+        for (int u = 0; u < 8; ++u) v[u] = hw[2 * (j + u)];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) rank += v[u] < rec ? 1 : 0;
+        for (int u = 0; u < 8; ++u) rank += v[u] < rh ? 1 : 0;
       }
-      for (; j < b1; ++j) rank += rout[j] < rec ? 1 : 0;
+      for (; j < b1; ++j) rank += hw[2 * j] < rh ? 1 : 0;
       rin[b0 + rank] = rec;
     }
     __syncthreads();
@@ -1645,12 +1666,7 @@ __device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, un
       const Codec cd = make_codec(m);
       float S, W;
       State::load(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, S, W);
-      const int e = (int)(start[t + 1] - base);
-      for (int i = (int)(ex - base); i < e; ++i) {
-        float w, wd;
-        weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
-        fold_step(S, W, w, wd, m.maxW);
-      }
+      fold_run(S, W, rin, (int)(ex - base), (int)(start[t + 1] - base), m.maxW);
       State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, S, W);
       uniq++;
     }
@@ -1779,14 +1795,7 @@ __device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long
       if (t == 0) start[kVoxelsPerBlock] = tot3;
     }
     bitonic_sort_lds(rin, n2);
-    if (h) {
-      const int e = (int)start[t + 1];
-      for (int i = (int)start[t]; i < e; ++i) {
-        float w, wd;
-        weight_of(__uint_as_float((uint32_t)rin[i]), w, wd);
-        fold_step(S, W, w, wd, m.maxW);
-      }
-    }
+    if (h) fold_run(S, W, rin, (int)start[t], (int)start[t + 1], m.maxW);
     if (t == 0) lo_sh = hi;
     __syncthreads();
   }
