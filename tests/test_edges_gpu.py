@@ -509,9 +509,11 @@ def tier_scans(n, cols=64):
 
 
 @pytest.mark.parametrize("n,cols", [(n, 64) for n in (16, 17, 32, 33, 64, 65, 128, 129, 1024, 1025, 4096, 4097,
-                                                      8192, 8193, 20000)] + [(12606, 1)])
+                                                      8192, 8193, 20000)]
+                         + [(1025, 1), (3000, 1), (4000, 1), (4096, 2), (12606, 1)])
 def test_fold_tiers_at_their_bounds(eng, oracle_lib, n, cols):
-    # blocks beyond 4096 records are folded in voxel ranges of about 2048 records; (12606, 1) puts 2101
+    # (1025, 1) .. (4096, 2) put 171..667 records in each of a few voxels of a 4096-record counting tile.
+    # Blocks beyond 4096 records are folded in voxel ranges of about 2048 records; (12606, 1) puts 2101
     # records in each of six voxels, past the range bound, which takes the merge path instead
     scans = tier_scans(n, cols)
     pool = workloads.Pool([(p, s, 0) for p, s in scans])
