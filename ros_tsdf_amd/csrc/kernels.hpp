@@ -98,6 +98,40 @@ __device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
   return v;
 }
 
+// Inclusive scan across the wave (lane i gets v[0] + ... + v[i]) in DPP
+// moves between registers, with no LDS round trip: Hillis-Steele steps
+// inside each row of 16 lanes (row_shr 1, 2, 4, 8; a lane whose source is
+// outside its row adds 0), then the total of row 0 into row 1 and of row 2
+// into row 3 (row_bcast15 on rows 1 and 3), then the total of rows 0-1 into
+// rows 2 and 3 (row_bcast31).  Rows a step leaves out add 0.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kCtrl, kRowMask, 0xf, true);
+}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  x += dpp_u32<0x111, 0xf>(x);  // row_shr:1
+  x += dpp_u32<0x112, 0xf>(x);  // row_shr:2
+  x += dpp_u32<0x114, 0xf>(x);  // row_shr:4
+  x += dpp_u32<0x118, 0xf>(x);  // row_shr:8
+  x += dpp_u32<0x142, 0xa>(x);  // row_bcast:15
+  x += dpp_u32<0x143, 0xc>(x);  // row_bcast:31
+  return x;
+}
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
+  const uint32_t lo = dpp_u32<kCtrl, kRowMask>((uint32_t)x), hi = dpp_u32<kCtrl, kRowMask>((uint32_t)(x >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned long long wave_incl_scan64(unsigned long long x) {
+  x += dpp_u64<0x111, 0xf>(x);
+  x += dpp_u64<0x112, 0xf>(x);
+  x += dpp_u64<0x114, 0xf>(x);
+  x += dpp_u64<0x118, 0xf>(x);
+  x += dpp_u64<0x142, 0xa>(x);
+  x += dpp_u64<0x143, 0xc>(x);
+  return x;
+}
+
 // The workgroup's row of pass statistics in region `region`: each thread's
 // counts v[0..N) summed over the workgroup.  Every workgroup writes its row,
 // zeros included, so rows need no clearing.  All threads must call this.
@@ -435,7 +469,12 @@ __device__ __forceinline__ uint32_t lds_hash(unsigned long long k) {
 static_assert(kBinSlots == 1024, "lds_hash yields 10 bits");
 
 // Slot of key k in the workgroup's table, inserting it when absent (-1 when
-// the table is full).  *added is set when this call inserted it.
+// the table is full).  *added is set when this call inserted it.  Probes
+// follow the triangular sequence h, h+1, h+3, h+6, ... (every slot of the
+// power-of-two table once), which breaks up the runs that the blocks of
+// neighbouring rays form under linear probing.  Each probe reads the slot
+// first: most lookups find their key, and many lanes reading one slot is a
+// broadcast where a compare-and-swap would serialise them.
 __device__ __forceinline__ int lds_find_or_add(unsigned long long* tkey, unsigned long long k, bool* added = nullptr) {
   uint32_t h = lds_hash(k);
   for (int probe = 0; probe < 64; ++probe) {
@@ -446,7 +485,23 @@ __device__ __forceinline__ int lds_find_or_add(unsigned long long* tkey, unsigne
       if (old == kEmptyKey && added) *added = true;
       if (old == kEmptyKey || old == k) return (int)h;
     }
-    h = (h + 1) & (kBinSlots - 1);
+    h = (h + (uint32_t)probe + 1u) & (kBinSlots - 1);
+  }
+  return -1;
+}
+// The same lookup for the block walks, which mostly insert: each probe is
+// one compare-and-swap, whose returned value also answers a lookup (one LDS
+// round trip per probe instead of a read and then the swap).
+__device__ __forceinline__ int lds_add(unsigned long long* tkey, unsigned long long k, bool* added) {
+  uint32_t h = lds_hash(k);
+  for (int probe = 0; probe < 64; ++probe) {
+    const unsigned long long old = atomicCAS(&tkey[h], kEmptyKey, k);
+    if (old == kEmptyKey) {
+      *added = true;
+      return (int)h;
+    }
+    if (old == k) return (int)h;
+    h = (h + (uint32_t)probe + 1u) & (kBinSlots - 1);
   }
   return -1;
 }
@@ -638,7 +693,7 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched 
           const int nb = walk_cells(a, b, m.BL, m.sc_b, m.isc_b, m.half_BL, m.max_blk, tb, [&](int32_t k0, int32_t k1, int32_t k2) {
             const int32_t kb[3] = {k0, k1, k2};
             bool added = false;
-            const int slot = lds_find_or_add(tkey, lds_block_key(sd.map, kb), &added);
+            const int slot = lds_add(tkey, lds_block_key(sd.map, kb), &added);
             if (added) locc[atomicAdd(&n_occ, 1)] = (uint16_t)slot;  // the table's list
             if (slot < 0) {
               bool ins;
@@ -849,12 +904,7 @@ template <int NW = 0>
 __device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* sh, uint32_t& total) {
   const int lane = lane_id();
   const int w = threadIdx.x >> 6;
-  uint32_t x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
+  const uint32_t x = wave_incl_scan(v);
   if (lane == 63) sh[w] = x;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -880,12 +930,7 @@ __device__ __forceinline__ unsigned long long block_scan_excl64(unsigned long lo
                                                                  unsigned long long& total) {
   const int lane = lane_id();
   const int w = threadIdx.x >> 6;
-  unsigned long long x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const unsigned long long y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
+  const unsigned long long x = wave_incl_scan64(v);
   if (lane == 63) sh[w] = x;
   __syncthreads();
   if (threadIdx.x == 0) {
