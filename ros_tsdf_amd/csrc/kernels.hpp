@@ -60,44 +60,6 @@ __host__ __device__ constexpr int fold_class(int n) {
 // ---- wave / workgroup helpers -------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
-// All lanes of the wave must call this.  Returns the slot of this lane's item
-// (or -1 when pred is false) in a list whose length is *ctr.
-__device__ __forceinline__ int wave_append(int32_t* ctr, bool pred) {
-  const unsigned long long m = __ballot(pred);
-  if (m == 0) return -1;
-  const int lane = lane_id();
-  const int leader = __ffsll((long long)m) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(ctr, (int)__popcll(m));
-  base = __shfl(base, leader);
-  const int rank = (int)__popcll(m & ((1ull << lane) - 1ull));
-  return pred ? base + rank : -1;
-}
-
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-// Statistics: each wave adds its sum into one of kPartSlots partial slots
-// (spread by wave, so the adds do not pile up on one address); the pass's
-// last kernel reduces the slots into EngCtr::stats.
-__device__ __forceinline__ void stat_add(const Dev& D, int stat, unsigned long long v) {
-  const unsigned long long s = wave_sum(v);
-  if (lane_id() == 0 && s) {
-    const int slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kPartSlots - 1);
-    atomicAdd(&D.part[stat * kPartSlots + slot], s);
-  }
-}
-
-
-__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
 // Inclusive scan across the wave (lane i gets v[0] + ... + v[i]) in DPP
 // moves between registers, with no LDS round trip: Hillis-Steele steps
 // inside each row of 16 lanes (row_shr 1, 2, 4, 8; a lane whose source is
@@ -131,6 +93,88 @@ __device__ __forceinline__ unsigned long long wave_incl_scan64(unsigned long lon
   x += dpp_u64<0x143, 0xc>(x);
   return x;
 }
+
+// Lane i's copy of x from lane i ^ J (J a power of two below 64), every lane
+// active, in VALU lane moves instead of an LDS permute: DPP quad_perm for 1
+// and 2, a quad mirror then row_half_mirror for 4 (i ^ 3 ^ 7), row_ror:8 for
+// 8, and gfx950's permlane16/32 swaps for 16 and 32.  A swap leaves each
+// lane its own value in one result and lane i ^ J's in the other, so their
+// sum minus x is the latter (exact in unsigned arithmetic).
+template <int J>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t x) {
+  if constexpr (J == 1) return dpp_u32<0xB1, 0xf>(x);
+  else if constexpr (J == 2) return dpp_u32<0x4E, 0xf>(x);
+  else if constexpr (J == 4) return dpp_u32<0x141, 0xf>(dpp_u32<0x1B, 0xf>(x));
+  else if constexpr (J == 8) return dpp_u32<0x128, 0xf>(x);
+  else if constexpr (J == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return r[0] + r[1] - x;
+  } else {
+    static_assert(J == 32, "a lane distance below 64");
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return r[0] + r[1] - x;
+  }
+}
+template <int J>
+__device__ __forceinline__ unsigned long long xor_lane64(unsigned long long v) {
+  return ((unsigned long long)xor_lane<J>((uint32_t)(v >> 32)) << 32) | xor_lane<J>((uint32_t)v);
+}
+// The same with j known only after unrolling (the switch folds away).
+__device__ __forceinline__ unsigned long long xor_lane64(unsigned long long v, int j) {
+  switch (j) {
+    case 1: return xor_lane64<1>(v);
+    case 2: return xor_lane64<2>(v);
+    case 4: return xor_lane64<4>(v);
+    case 8: return xor_lane64<8>(v);
+    case 16: return xor_lane64<16>(v);
+    default: return xor_lane64<32>(v);
+  }
+}
+// Lane i's copy of x from lane i + 1 (wave_shl:1; the last lane gets 0), and
+// from lane i - 1 (wave_shr:1; lane 0 gets 0).
+__device__ __forceinline__ uint32_t next_lane(uint32_t x) { return dpp_u32<0x130, 0xf>(x); }
+__device__ __forceinline__ float next_lane(float x) { return __uint_as_float(next_lane(__float_as_uint(x))); }
+__device__ __forceinline__ unsigned long long prev_lane64(unsigned long long v) {
+  return ((unsigned long long)dpp_u32<0x138, 0xf>((uint32_t)(v >> 32)) << 32) | dpp_u32<0x138, 0xf>((uint32_t)v);
+}
+
+// All lanes of the wave must call this.  Returns the slot of this lane's item
+// (or -1 when pred is false) in a list whose length is *ctr.
+__device__ __forceinline__ int wave_append(int32_t* ctr, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (m == 0) return -1;
+  const int lane = lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(ctr, (int)__popcll(m));
+  base = __builtin_amdgcn_readlane(base, leader);
+  const int rank = (int)__popcll(m & ((1ull << lane) - 1ull));
+  return pred ? base + rank : -1;
+}
+
+// Sums over the wave (every lane active; every lane gets the sum).
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+  const unsigned long long x = wave_incl_scan64(v);
+  return ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(x >> 32), 63) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// Statistics: each wave adds its sum into one of kPartSlots partial slots
+// (spread by wave, so the adds do not pile up on one address); the pass's
+// last kernel reduces the slots into EngCtr::stats.
+__device__ __forceinline__ void stat_add(const Dev& D, int stat, unsigned long long v) {
+  const unsigned long long s = wave_sum(v);
+  if (lane_id() == 0 && s) {
+    const int slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kPartSlots - 1);
+    atomicAdd(&D.part[stat * kPartSlots + slot], s);
+  }
+}
+
+
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(v), 63);
+}
+
 
 // The workgroup's row of pass statistics in region `region`: each thread's
 // counts v[0..N) summed over the workgroup.  Every workgroup writes its row,
@@ -395,11 +439,11 @@ __device__ __forceinline__ int assign_heap(const Dev& D, bool active, uint32_t e
   unsigned long long pending = __ballot(active);
   while (pending) {
     const int leader = __ffsll((long long)pending) - 1;
-    const int lm = __shfl(ms, leader);
+    const int lm = __builtin_amdgcn_readlane(ms, leader);
     const unsigned long long grp = __ballot(active && ms == lm);
     int base = 0;
     if (lane == leader) base = atomicAdd(&D.mctr[lm].blocks, (int)__popcll(grp));
-    base = __shfl(base, leader);
+    base = __builtin_amdgcn_readlane(base, leader);
     if (grp >> lane & 1ull) held = base + (int)__popcll(grp & below);
     pending &= ~grp;
   }
@@ -422,7 +466,7 @@ __device__ __forceinline__ int assign_heap(const Dev& D, bool active, uint32_t e
     const int failed = min(cnt, cnt - max(top, 0));
     if (failed > 0) atomicAdd(&D.ctr->heap_top, failed);
   }
-  top = __shfl(top, leader);
+  top = __builtin_amdgcn_readlane(top, leader);
   const int idx = top - cnt + (int)__popcll(want & below);
   if (ok && idx < 0) {
     atomicSub(&D.mctr[ms].blocks, 1);
@@ -1370,12 +1414,12 @@ __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int 
     for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
       for (int j = k >> 1; j > 0; j >>= 1) {
-        const unsigned long long o = __shfl_xor(v, j);
+        const unsigned long long o = xor_lane64(v, j);
         const bool keep_min = ((gl & j) == 0) == ((gl & k) == 0);
         v = keep_min ? (o < v ? o : v) : (o > v ? o : v);
       }
     }
-    const unsigned long long prev = __shfl_up(v, 1);
+    const unsigned long long prev = prev_lane64(v);
     const bool head = gl < n && (gl == 0 || rec_voxel(prev) != rec_voxel(v));
     const unsigned long long heads = __ballot(head);
     const int gend = grp * G + n;  // first lane past this group's records
@@ -1390,13 +1434,19 @@ __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int 
     const int64_t vi = (int64_t)d.z * kVoxelsPerBlock + rec_voxel(v);
     float S = 0.f, W = 0.f;
     if (head) State::load(D.heap, vi, cd, S, W);
-    int maxlen = len;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, o));
-    for (int r = 0; r < maxlen; ++r) {
-      const float wr = __shfl(w, lane + r);
-      const float wdr = __shfl(wd, lane + r);
+    uint32_t maxlen = (uint32_t)len;
+    maxlen = max(maxlen, xor_lane<32>(maxlen));
+    maxlen = max(maxlen, xor_lane<16>(maxlen));
+    maxlen = max(maxlen, xor_lane<8>(maxlen));
+    maxlen = max(maxlen, xor_lane<4>(maxlen));
+    maxlen = max(maxlen, xor_lane<2>(maxlen));
+    maxlen = max(maxlen, xor_lane<1>(maxlen));
+    // step r: lane i holds the weight terms of lane i + r
+    float wr = w, wdr = wd;
+    for (int r = 0; r < (int)maxlen; ++r) {
       if (r < len) fold_step(S, W, wr, wdr, maxW);
+      wr = next_lane(wr);
+      wdr = next_lane(wdr);
     }
     if (head) {
       State::store(D.heap, vi, cd, S, W);
@@ -1441,7 +1491,7 @@ __device__ __forceinline__ void fold_wave_block(const Dev& D, const int4 d, unsi
       } else {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const unsigned long long o = __shfl_xor(v[r], j);
+          const unsigned long long o = xor_lane64(v[r], j);
           const bool up = (((64 * r) | lane) & k) == 0;
           const bool keep_min = ((lane & j) == 0) == up;
           v[r] = keep_min ? (o < v[r] ? o : v[r]) : (o > v[r] ? o : v[r]);
