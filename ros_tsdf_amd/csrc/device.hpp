@@ -172,6 +172,8 @@ struct StateF32 {
   __device__ static __forceinline__ void store(void* heap, int64_t i, const Codec&, float S, float W) {
     reinterpret_cast<float2*>(heap)[i] = make_float2(S, W);
   }
+  // the state as a store and a load would leave it
+  __device__ static __forceinline__ void round(const Codec&, float&, float&) {}
   __device__ static __forceinline__ bool nonzero(const void* heap, int64_t i) {
     return reinterpret_cast<const float2*>(heap)[i].y != 0.0f;
   }
@@ -190,6 +192,10 @@ struct StateF16 {
     const unsigned short s = __half_as_ushort(__float2half_rn(S * c.inv_T));
     const unsigned short w = __half_as_ushort(__float2half_rn(W));
     reinterpret_cast<uint32_t*>(heap)[i] = (uint32_t)s | ((uint32_t)w << 16);
+  }
+  __device__ static __forceinline__ void round(const Codec& c, float& S, float& W) {
+    S = __half2float(__float2half_rn(S * c.inv_T)) * c.T;
+    W = __half2float(__float2half_rn(W));
   }
   __device__ static __forceinline__ bool nonzero(const void* heap, int64_t i) {
     return (reinterpret_cast<const uint32_t*>(heap)[i] >> 16) != 0u;
@@ -213,6 +219,12 @@ struct StateF16Q {
     // a touched voxel keeps a nonzero weight
     const uint32_t w = W > 0.0f ? max((uint32_t)q, 1u) : 0u;
     reinterpret_cast<uint32_t*>(heap)[i] = (uint32_t)s | (w << 16);
+  }
+  __device__ static __forceinline__ void round(const Codec& c, float& S, float& W) {
+    const float q = fminf(fmaxf(rintf(W * c.wq), 0.0f), 65535.0f);
+    const uint32_t w = W > 0.0f ? max((uint32_t)q, 1u) : 0u;
+    S = __half2float(__float2half_rn(S * c.inv_T)) * c.T;
+    W = (float)w * c.inv_wq;
   }
   __device__ static __forceinline__ bool nonzero(const void* heap, int64_t i) {
     return (reinterpret_cast<const uint32_t*>(heap)[i] >> 16) != 0u;

@@ -28,6 +28,8 @@
 //          -> scan -> k_pub_write (wave ballot compaction)
 // gc:      k_gc_mark -> k_gc_zero -> k_gc_chains
 #pragma once
+#include <type_traits>
+
 #include "device.hpp"
 
 namespace tsdfhip {
@@ -40,7 +42,9 @@ constexpr int kScanTile = 1024;  // elements per workgroup in the scan kernels
 
 // Size classes of touched blocks (records per block): the fold kernel and
 // the number of blocks folded together differ per class.
-enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WGL, FC_WG, FC_WG2, FC_HUGE, FC_COUNT };
+// FC_DEFER holds blocks of FC_WG that k_fold_a hands to k_fold_b (a group of
+// theirs would cross max_weight); no block is filed there by size.
+enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WGL, FC_WG, FC_WG2, FC_HUGE, FC_DEFER, FC_COUNT };
 constexpr int kWgSplit = 128;  // FC_WGL: 65..128 records (a wave each), FC_WG: 129..kCountTileS
 static_assert(FC_COUNT <= (int)(sizeof(EngCtr::fold_n) / sizeof(int32_t)), "fold class counters");
 
@@ -48,7 +52,7 @@ static_assert(FC_COUNT <= (int)(sizeof(EngCtr::fold_n) / sizeof(int32_t)), "fold
 // blocks and `records` records: every block of class c holds at least lo
 // records (the class's lower size bound).
 __host__ __device__ constexpr int64_t fold_class_cap(int c, int64_t blocks, int64_t records) {
-  const int64_t lo = c == FC_16 ? 1 : c == FC_32 ? 17 : c == FC_64 ? 33 : c == FC_WGL ? 65 : c == FC_WG ? kWgSplit + 1 : c == FC_WG2 ? kCountTileS + 1 : kCountTile + 1;
+  const int64_t lo = c == FC_16 ? 1 : c == FC_32 ? 17 : c == FC_64 ? 33 : c == FC_WGL ? 65 : (c == FC_WG || c == FC_DEFER) ? kWgSplit + 1 : c == FC_WG2 ? kCountTileS + 1 : kCountTile + 1;
   const int64_t r = records / lo + 1;
   return r < blocks ? r : blocks;
 }
@@ -134,6 +138,9 @@ __device__ __forceinline__ unsigned long long xor_lane64(unsigned long long v, i
 // from lane i - 1 (wave_shr:1; lane 0 gets 0).
 __device__ __forceinline__ uint32_t next_lane(uint32_t x) { return dpp_u32<0x130, 0xf>(x); }
 __device__ __forceinline__ float next_lane(float x) { return __uint_as_float(next_lane(__float_as_uint(x))); }
+__device__ __forceinline__ unsigned long long next_lane64(unsigned long long v) {
+  return ((unsigned long long)next_lane((uint32_t)(v >> 32)) << 32) | next_lane((uint32_t)v);
+}
 __device__ __forceinline__ unsigned long long prev_lane64(unsigned long long v) {
   return ((unsigned long long)dpp_u32<0x138, 0xf>((uint32_t)(v >> 32)) << 32) | dpp_u32<0x138, 0xf>((uint32_t)v);
 }
@@ -1283,30 +1290,317 @@ __device__ __forceinline__ void weight_of(float d, float& w, float& wd) {
   wd = w * d;
 }
 
-// Folds the run rin[b..e) of one voxel's records into (S, W), in order.  The
-// distances are read from LDS four at a time and their weights computed
-// before the four serial fold steps, so the chain of steps does not wait for
-// an LDS read at every step.
-__device__ __forceinline__ void fold_run(float& S, float& W, const unsigned long long* rin, int b, int e,
-                                         float maxW) {
-  const uint32_t* lo = reinterpret_cast<const uint32_t*>(rin);  // (the distance is a record's low word)
-  int i = b;
-  for (; i + 4 <= e; i += 4) {
-    float w[4], wd[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) weight_of(__uint_as_float(lo[2 * (i + u)]), w[u], wd[u]);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) fold_step(S, W, w[u], wd[u], maxW);
-  }
-  for (; i < e; ++i) {
-    float w, wd;
-    weight_of(__uint_as_float(lo[2 * i]), w, wd);
-    fold_step(S, W, w, wd, maxW);
-  }
-}
-
 __device__ __forceinline__ int rec_voxel(unsigned long long r) { return (int)(r >> 55); }
 __device__ __forceinline__ int rec_point(unsigned long long r) { return (int)((r >> 32) & ((1u << kPointIdxBits) - 1u)); }
+__device__ __forceinline__ float rec_dist(unsigned long long r) { return __uint_as_float((uint32_t)r); }
+
+// ---- the fold of one voxel's updates in a pass ---------------------------------------------
+// Update by update (H:637-654), S = (W S + w d)/(W + w) and W = min(W + w, maxW).
+// While W + w stays within maxW, n updates in a row give the weighted mean
+//   S_n = (W_0 S_0 + sum w_i d_i) / (W_0 + sum w_i),   W_n = W_0 + sum w_i,
+// whatever their order.  A pass therefore folds the updates of each (voxel,
+// scan) pair -- a group -- as one such mean: each update's w and w d (FP32,
+// as fold_step forms them) become fixed-point integers with 21 fractional
+// bits (rounded to nearest: at most 2^-22 off, far below the fold's
+// tolerance), their sums are exact and independent of the order in which
+// updates arrive, and the mean is formed once.  A group whose weight would
+// cross maxW is folded in ascending point index: as one mean up to the update
+// that would cross it, then update by update.  Groups are applied in scan
+// order and the state is rounded to its stored format between them, so a
+// batch of scans gives the maps that the same scans give one pass each.
+// 2^21: one unit of weight.  Both terms of an update are below 2^21 in
+// magnitude, so the sums of up to 1024 updates fit 32-bit accumulators.
+constexpr float kFixOne = 2097152.0f;
+constexpr int kFixSum32 = 1024;  // updates whose sums fit 32 bits
+
+__device__ __forceinline__ void fixed_terms(float d, uint32_t& wq, int32_t& wdq) {
+  float w, wd;
+  weight_of(d, w, wd);
+  wq = max(__float2uint_rn(w * kFixOne), 1u);  // (an update always carries weight)
+  wdq = __float2int_rn(wd * kFixOne);
+}
+// Fixed-point weight that a state of weight W can take before it reaches maxW.
+__device__ __forceinline__ unsigned long long weight_room(float W, float maxW) {
+  const double r = ((double)maxW - (double)W) * (double)kFixOne;
+  return r <= 0.0 ? 0ull : r >= 1.8e19 ? ~0ull : (unsigned long long)r;
+}
+// A fixed-point sum as FP32: its high and low words converted and joined
+// by one fused multiply-add.  A negative sum is converted by its magnitude:
+// its two's-complement low word lies near 2^32, where FP32 keeps only 24 of
+// its bits.
+__device__ __forceinline__ float fixed_to_float(unsigned long long v) {
+  return __builtin_fmaf((float)(uint32_t)(v >> 32), 4294967296.0f, (float)(uint32_t)v) * (1.0f / kFixOne);
+}
+__device__ __forceinline__ float fixed_to_float(long long v) {
+  const float f = fixed_to_float((unsigned long long)(v < 0 ? -v : v));
+  return v < 0 ? -f : f;
+}
+// One group's sums applied as a weighted mean (the caller has checked the
+// room), with the FP32 operations and hardware reciprocal of fold_step; wf
+// is fixed_to_float(sw).
+__device__ __forceinline__ void apply_sums(float& S, float& W, float wf, long long swd) {
+  const float Wn = W + wf;
+  S = (W * S + fixed_to_float(swd)) * __builtin_amdgcn_rcpf(Wn);
+  W = Wn;
+}
+__device__ __forceinline__ void apply_sums(float& S, float& W, unsigned long long sw, long long swd) {
+  apply_sums(S, W, fixed_to_float(sw), swd);
+}
+// Whether a group of fixed-point weight sw (wf as FP32) fits below maxW:
+// sw <= weight_room(W, maxW).  The FP32 test first: W + wf within 0.999 maxW
+// (its rounding is far below that margin) implies it, so the exact test in
+// FP64 only runs near maxW.
+__device__ __forceinline__ bool group_fits(float W, unsigned long long sw, float wf, float maxW) {
+  return W + wf <= 0.999f * maxW || sw <= weight_room(W, maxW);
+}
+
+// The pass's scan starts (soff[ns] = points in the pass) and maps, staged in
+// LDS by the fold workgroups: a record's group is the scan of its point.
+struct PassScans {
+  int32_t off[kMaxScansPerBatch + 1];
+  uint8_t map[kMaxScansPerBatch];
+  int32_t ns;
+};
+__device__ __forceinline__ void stage_scans(const Dev& D, PassScans& ps) {
+  const PassHdr hdr = *D.hdr;
+  for (int s = threadIdx.x; s < hdr.nscans; s += blockDim.x) {
+    const ScanDev sd = D.scans[s];
+    ps.off[s] = sd.off;
+    ps.map[s] = (uint8_t)sd.map;
+  }
+  if (threadIdx.x == 0) {
+    ps.off[hdr.nscans] = hdr.total;
+    ps.ns = hdr.nscans;
+  }
+}
+// The scan of point p (scans are contiguous and in pass order), searched
+// from scan `from` on (p lies in it or after it): a few steps forward, where
+// the next group of a map usually is, else by bisection.
+__device__ __forceinline__ int scan_in_lds(const PassScans& ps, int p, int from) {
+  int s = from;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (ps.off[s + 1] <= p) ++s;
+  if (ps.off[s + 1] > p) return s;
+  int lo = s, hi = ps.ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ps.off[mid] <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The ordered fold of one voxel's updates, fed in ascending point index
+// (possibly over several calls, as the point rounds do).
+template <class State>
+struct OrderedFold {
+  float S, W;
+  unsigned long long room, sw;  // the open group: room left below maxW, its pending weight sum
+  long long swd;
+  int32_t gend;                 // first point past the open group (-1: none open yet)
+  int32_t gscan;                // its scan
+  bool seq;                     // the open group crossed maxW: update by update from here on
+
+  __device__ __forceinline__ void begin(float S0, float W0) {
+    S = S0;
+    W = W0;
+    room = 0;
+    sw = 0;
+    swd = 0;
+    gend = -1;
+    gscan = 0;
+    seq = false;
+  }
+  __device__ __forceinline__ void close() {
+    if (!seq && sw != 0) apply_sums(S, W, sw, swd);
+    sw = 0;
+    swd = 0;
+  }
+  __device__ __forceinline__ void add(unsigned long long rec, float maxW, const Codec& cd, const PassScans& ps) {
+    uint32_t wq;
+    int32_t wdq;
+    fixed_terms(rec_dist(rec), wq, wdq);
+    add_terms(rec, wq, wdq, maxW, cd, ps);
+  }
+  // opens the group of point p: the previous one is applied and rounded to
+  // the stored format
+  __device__ __forceinline__ void open_group(int p, float maxW, const Codec& cd, const PassScans& ps) {
+    if (gend >= 0) {
+      close();
+      State::round(cd, S, W);
+    }
+    if (ps.ns != 1) gscan = scan_in_lds(ps, p, gscan);
+    gend = ps.off[gscan + 1];
+    room = weight_room(W, maxW);
+    seq = false;
+  }
+  // an update of a group past maxW: the pending sums applied, then update by update
+  __device__ __forceinline__ void step_seq(unsigned long long rec, float maxW) {
+    if (!seq) {
+      close();
+      seq = true;
+    }
+    float w, wd;
+    weight_of(rec_dist(rec), w, wd);
+    fold_step(S, W, w, wd, maxW);
+  }
+  // the same with the record's fixed-point terms formed by the caller
+  __device__ __forceinline__ void add_terms(unsigned long long rec, uint32_t wq, int32_t wdq, float maxW,
+                                            const Codec& cd, const PassScans& ps) {
+    if (rec_point(rec) >= gend) open_group(rec_point(rec), maxW, cd, ps);
+    if (!seq && sw + wq <= room) {
+      sw += wq;
+      swd += wdq;
+      return;
+    }
+    step_seq(rec, maxW);
+  }
+};
+
+// The groups of one block's map: bnd[j] is the start of the map's j-th scan
+// in the pass; returns their number G.  Every wave counts them (each reads
+// the pass's scans from LDS); wave 0 files the starts.
+__device__ __forceinline__ int map_groups(const PassScans& ps, int m, int32_t* bnd) {
+  int G = 0;
+  const int lane = lane_id();
+  const bool filer = threadIdx.x < 64;
+  for (int s0 = 0; s0 < ps.ns; s0 += 64) {
+    const int s = s0 + lane;
+    const bool mine = s < ps.ns && ps.map[s] == m;
+    const unsigned long long b = __ballot(mine);
+    if (filer && mine) bnd[G + (int)__popcll(b & ((1ull << lane) - 1ull))] = ps.off[s];
+    G += (int)__popcll(b);
+  }
+  return G;
+}
+// Group (rank of the scan within the block's map) of point p; b1..b3 are
+// bnd[1..3] (past every point where G is smaller), bnd in LDS for G > 4.
+__device__ __forceinline__ int group_of(int p, int G, int32_t b1, int32_t b2, int32_t b3, const int32_t* bnd) {
+  if (G <= 4) return (p >= b1 ? 1 : 0) + (p >= b2 ? 1 : 0) + (p >= b3 ? 1 : 0);
+  int lo = 0, hi = G - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bnd[mid] <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One block by sums: each voxel's group sums in LDS accumulators (acc: gw
+// windows of 512 {weight, weighted distance} pairs, 64-bit when kWide, else
+// 32-bit for blocks of at most kFixSum32 records; groups wb .. wb + gw - 1
+// per turn over the block's records), then each voxel's groups applied in
+// order by the voxel's thread.  A block of at most 4 NT records keeps them
+// in registers across the turns; a larger one reads them again per turn.
+// Returns true, with nothing stored, when some group would cross maxW: the
+// block then takes an ordered fold.  bnd: LDS for the map's group starts
+// (kMaxScansPerBatch entries); flag: an LDS word.
+template <class State, int NT, bool kWide>
+__device__ bool fold_sum_block(const Dev& D, const int4 d, void* acc, int gw, const PassScans& ps,
+                               int32_t* bnd, int* flag, unsigned long long& uniq) {
+  using Pair = typename std::conditional<kWide, ulonglong2, uint2>::type;
+  Pair* const accp = reinterpret_cast<Pair*>(acc);
+  constexpr int kVpt = NT >= kVoxelsPerBlock ? 1 : kVoxelsPerBlock / NT;  // voxels per thread
+  const int t = threadIdx.x;
+  const int c = d.y;
+  const unsigned long long* sg = D.seg + (uint32_t)d.x;
+  const int G = ps.ns == 1 ? 1 : map_groups(ps, d.w, bnd);
+  const MapDev& m = D.maps[d.w];
+  const float maxW = m.maxW;
+  const Codec cd = make_codec(m);
+  float S[kVpt], W[kVpt];
+  bool seen[kVpt];
+#pragma unroll
+  for (int u = 0; u < kVpt; ++u) {
+    S[u] = 0.f;
+    W[u] = 0.f;
+    seen[u] = false;
+  }
+  const bool held = c <= 4 * NT;
+  unsigned long long r[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) r[u] = held && t + u * NT < c ? sg[t + u * NT] : 0ull;
+  for (int wb = 0;; wb += gw) {
+    const int nw = min(gw, G - wb);
+    for (int x = t; x < nw * kVoxelsPerBlock; x += NT) accp[x] = Pair{0, 0};
+    __syncthreads();
+    // (every wave has read the previous block's flag: it passed this barrier)
+    if (t == 0) *flag = 0;
+    const int32_t b1 = G > 1 ? bnd[1] : INT32_MAX, b2 = G > 2 ? bnd[2] : INT32_MAX, b3 = G > 3 ? bnd[3] : INT32_MAX;
+    for (int x0 = t; x0 < c; x0 += 4 * NT) {
+      if (!held) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] = x0 + u * NT < c ? sg[x0 + u * NT] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (x0 + u * NT >= c) continue;
+        const int g = (G == 1 ? 0 : group_of(rec_point(r[u]), G, b1, b2, b3, bnd)) - wb;
+        if ((unsigned)g >= (unsigned)nw) continue;
+        uint32_t wq;
+        int32_t wdq;
+        fixed_terms(rec_dist(r[u]), wq, wdq);
+        Pair* a = accp + g * kVoxelsPerBlock + rec_voxel(r[u]);
+        if constexpr (kWide) {
+          atomicAdd(&a->x, (unsigned long long)wq);
+          atomicAdd(&a->y, (unsigned long long)(long long)wdq);
+        } else {
+          atomicAdd(&a->x, wq);
+          atomicAdd(&a->y, (uint32_t)wdq);
+        }
+      }
+    }
+    __syncthreads();
+    // each voxel's groups of this turn, in order: the states of the voxels
+    // first touched now are requested together, then the groups applied
+    bool clamp = false;
+#pragma unroll
+    for (int u = 0; u < kVpt; ++u) {
+      const int l = t + NT * u;
+      if (l >= kVoxelsPerBlock) continue;
+      bool any = false;
+      for (int g = 0; g < nw; ++g) any = any || accp[g * kVoxelsPerBlock + l].x != 0;
+      if (any && !seen[u]) State::load(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, S[u], W[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kVpt; ++u) {
+      const int l = t + NT * u;
+      if (l >= kVoxelsPerBlock) continue;
+      for (int g = 0; g < nw; ++g) {
+        const Pair ap = accp[g * kVoxelsPerBlock + l];
+        const unsigned long long sw = ap.x;
+        const long long swd = kWide ? (long long)ap.y : (long long)(int32_t)ap.y;
+        if (sw == 0) continue;
+        if (seen[u]) State::round(cd, S[u], W[u]);
+        seen[u] = true;
+        const float wf = fixed_to_float(sw);
+        if (group_fits(W[u], sw, wf, maxW)) apply_sums(S[u], W[u], wf, swd);
+        else clamp = true;
+      }
+    }
+    if (clamp) *flag = 1;
+    __syncthreads();
+    if (*flag) return true;
+    if (wb + gw >= G) break;
+  }
+#pragma unroll
+  for (int u = 0; u < kVpt; ++u) {
+    const int l = t + NT * u;
+    if (l < kVoxelsPerBlock && seen[u]) {
+      State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, S[u], W[u]);
+      uniq++;
+    }
+  }
+  return false;
+}
+
+// The ordered fold of the run rin[b..e) of one voxel's records (ascending point).
+template <class State>
+__device__ __forceinline__ void fold_run(OrderedFold<State>& of, const unsigned long long* rin, int b, int e,
+                                         float maxW, const Codec& cd, const PassScans& ps) {
+  for (int i = b; i < e; ++i) of.add(rin[i], maxW, cd, ps);
+}
 
 
 // Record segments and fold descriptors of the touched blocks.  Per tile of
@@ -1396,7 +1690,8 @@ __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
 // terms from the following lanes.
 // One wave folds the class-G descriptors [q_begin, q_end), 64/G at a time.
 template <class State, int G>
-__device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int q_end, unsigned long long& uniq) {
+__device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int q_end, const PassScans& ps,
+                                                 unsigned long long& uniq) {
   constexpr int kPer = 64 / G;
   constexpr int kCls = G == 16 ? FC_16 : G == 32 ? FC_32 : FC_64;
   const int lane = lane_id();
@@ -1426,14 +1721,17 @@ __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int 
     const unsigned long long above = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
     const int nh = above ? __ffsll((long long)above) - 1 : 64;
     const int len = head ? min(nh, gend) - lane : 0;
-    float w, wd;
-    weight_of(__uint_as_float((uint32_t)v), w, wd);
+    uint32_t wq;
+    int32_t wdq;
+    fixed_terms(rec_dist(v), wq, wdq);
     const MapDev& m = D.maps[d.w];
     const float maxW = m.maxW;
     const Codec cd = make_codec(m);
     const int64_t vi = (int64_t)d.z * kVoxelsPerBlock + rec_voxel(v);
-    float S = 0.f, W = 0.f;
-    if (head) State::load(D.heap, vi, cd, S, W);
+    float S0 = 0.f, W0 = 0.f;
+    if (head) State::load(D.heap, vi, cd, S0, W0);
+    OrderedFold<State> of;
+    of.begin(S0, W0);
     uint32_t maxlen = (uint32_t)len;
     maxlen = max(maxlen, xor_lane<32>(maxlen));
     maxlen = max(maxlen, xor_lane<16>(maxlen));
@@ -1441,15 +1739,35 @@ __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int 
     maxlen = max(maxlen, xor_lane<4>(maxlen));
     maxlen = max(maxlen, xor_lane<2>(maxlen));
     maxlen = max(maxlen, xor_lane<1>(maxlen));
-    // step r: lane i holds the weight terms of lane i + r
-    float wr = w, wdr = wd;
+    // step r: lane i holds the record and terms of lane i + r
+    unsigned long long vr = v;
+    uint32_t wqr = wq;
+    int32_t wdqr = wdq;
+    // the common turn (a pending group takes the update) is straight-line
+    // code; group changes and updates past maxW go through branches that a
+    // wave enters only when one of its lanes needs them
     for (int r = 0; r < (int)maxlen; ++r) {
-      if (r < len) fold_step(S, W, wr, wdr, maxW);
-      wr = next_lane(wr);
-      wdr = next_lane(wdr);
+      const bool act = r < len;
+      const int p = rec_point(vr);
+      const bool chg = act && p >= of.gend;
+      if (__ballot(chg)) {
+        if (chg) of.open_group(p, maxW, cd, ps);
+      }
+      const unsigned long long nsw = of.sw + wqr;
+      const bool fits = act && !of.seq && nsw <= of.room;
+      of.sw = fits ? nsw : of.sw;
+      of.swd = fits ? of.swd + wdqr : of.swd;
+      const bool slow = act && !fits;
+      if (__ballot(slow)) {
+        if (slow) of.step_seq(vr, maxW);
+      }
+      vr = next_lane64(vr);
+      wqr = next_lane(wqr);
+      wdqr = (int32_t)next_lane((uint32_t)wdqr);
     }
     if (head) {
-      State::store(D.heap, vi, cd, S, W);
+      of.close();
+      State::store(D.heap, vi, cd, of.S, of.W);
       uniq++;
     }
   }
@@ -1463,7 +1781,7 @@ __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int 
 // No workgroup barrier: the wave only waits for its own LDS traffic.
 template <class State, int R>
 __device__ __forceinline__ void fold_wave_block(const Dev& D, const int4 d, unsigned long long* strip,
-                                                unsigned long long& uniq) {
+                                                const PassScans& ps, unsigned long long& uniq) {
   constexpr int N = 64 * R;
   const int lane = lane_id();
   const int n = d.y;
@@ -1522,15 +1840,16 @@ __device__ __forceinline__ void fold_wave_block(const Dev& D, const int4 d, unsi
     const int l = rec_voxel(v[r]);
     int x = 64 * r + lane;
     unsigned long long cur = v[r];
+    OrderedFold<State> of;
+    of.begin(S[r], W[r]);
     for (;;) {
-      float w, wd;
-      weight_of(__uint_as_float((uint32_t)cur), w, wd);
-      fold_step(S[r], W[r], w, wd, maxW);
+      of.add(cur, maxW, cd, ps);
       if (++x >= n) break;
       cur = strip[x];
       if (rec_voxel(cur) != l) break;
     }
-    State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, S[r], W[r]);
+    of.close();
+    State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, of.S, of.W);
     uniq++;
   }
 }
@@ -1561,7 +1880,7 @@ __device__ __forceinline__ void bitonic_sort_lds(unsigned long long* a, int n) {
 template <class State, int TILE, int NT, class StartT = uint32_t>
 __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, unsigned long long* rin,
                                                  unsigned long long* rout, uint32_t* cnt, StartT* start,
-                                                 uint32_t* sh, unsigned long long& uniq) {
+                                                 uint32_t* sh, const PassScans& ps, unsigned long long& uniq) {
   static_assert(NT == 256 || NT == 512 || NT == 1024, "workgroup of 256, 512 or 1024 threads");
   constexpr int kPer = TILE / NT;                                         // records per thread
   constexpr int kVpt = NT >= kVoxelsPerBlock ? 1 : kVoxelsPerBlock / NT;  // voxels per thread
@@ -1658,12 +1977,13 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
 #pragma unroll
   for (int u = 0; u < kVpt; ++u) {
     const int l = threadIdx.x + NT * u;
-    if (l >= kVoxelsPerBlock) continue;
-    fold_run(S[u], W[u], rin, (int)start[l], (int)start[l + 1], maxW);
-    if (h[u]) {
-      State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, S[u], W[u]);
-      uniq++;
-    }
+    if (l >= kVoxelsPerBlock || !h[u]) continue;
+    OrderedFold<State> of;
+    of.begin(S[u], W[u]);
+    fold_run(of, rin, (int)start[l], (int)start[l + 1], maxW, cd, ps);
+    of.close();
+    State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, of.S, of.W);
+    uniq++;
   }
   __syncthreads();
 }
@@ -1685,7 +2005,8 @@ constexpr int kMaxRanges = 512;
 template <class State, int NT>
 __device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, unsigned long long* rin,
                                                   unsigned long long* rout, uint32_t* cnt, uint32_t* start,
-                                                  uint32_t* rstart, uint32_t* sh, unsigned long long& uniq) {
+                                                  uint32_t* rstart, uint32_t* sh, const PassScans& ps,
+                                                  unsigned long long& uniq) {
   static_assert(NT >= kVoxelsPerBlock, "a thread per voxel");
   static_assert(2 * kRangeSpan <= kCountTile, "a range fits the counting tile");
   __shared__ int too_big, nranges_sh, maxc_sh;
@@ -1761,8 +2082,11 @@ __device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, un
       const Codec cd = make_codec(m);
       float S, W;
       State::load(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, S, W);
-      fold_run(S, W, rin, (int)(ex - base), (int)(start[t + 1] - base), m.maxW);
-      State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, S, W);
+      OrderedFold<State> of;
+      of.begin(S, W);
+      fold_run(of, rin, (int)(ex - base), (int)(start[t + 1] - base), m.maxW, cd, ps);
+      of.close();
+      State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, of.S, of.W);
       uniq++;
     }
     __syncthreads();
@@ -1789,7 +2113,8 @@ constexpr int kRoundBinShift = 10;
 constexpr int kRoundBins = 1 << (kPointIdxBits - kRoundBinShift);  // first-level bins over the pass's points
 template <class State, int NT>
 __device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long* rin, unsigned long long* rout,
-                                  uint32_t* cnt, uint32_t* start, uint32_t* sh, unsigned long long& uniq) {
+                                  uint32_t* cnt, uint32_t* start, uint32_t* sh, const PassScans& ps,
+                                  unsigned long long& uniq) {
   static_assert(NT == (1 << kRoundBinShift), "a thread per point of a second-level bin");
   static_assert(NT >= kVoxelsPerBlock, "a thread per voxel");
   static_assert(kRoundBins * sizeof(uint32_t) <= kCountTile * sizeof(unsigned long long), "histogram fits in rout");
@@ -1810,6 +2135,8 @@ __device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long
   const bool h = t < kVoxelsPerBlock && cnt[t] != 0;
   float S = 0.f, W = 0.f;
   if (h) State::load(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, S, W);
+  OrderedFold<State> of;  // (kept across the rounds)
+  of.begin(S, W);
   constexpr int kPer = kRoundBins / NT;
   for (;;) {
     // level 1: records of points >= lo, per bin of 1024 points
@@ -1890,32 +2217,36 @@ __device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long
       if (t == 0) start[kVoxelsPerBlock] = tot3;
     }
     bitonic_sort_lds(rin, n2);
-    if (h) fold_run(S, W, rin, (int)start[t], (int)start[t + 1], m.maxW);
+    if (h) fold_run(of, rin, (int)start[t], (int)start[t + 1], m.maxW, cd, ps);
     if (t == 0) lo_sh = hi;
     __syncthreads();
   }
   if (h) {
-    State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, S, W);
+    of.close();
+    State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, of.S, of.W);
     uniq++;
   }
   __syncthreads();
 }
 
 // Fold, first launch.  Work items, in this order: the blocks of 129..1024
-// records (a workgroup each), those of 65..128 (a wave each), then chunks of
-// the small classes (one group of 64/G blocks per wave).  20.5 KB of LDS and at most
-// 72 VGPRs per thread let seven workgroups share a CU (the second launch
-// bound: seven waves per SIMD).  Workgroup q takes items q, q + grid, ...; with a
-// grid larger than what fits on the chip at once, the dispatcher starts a
-// new workgroup whenever one finishes, which balances the uneven items.
+// records (a workgroup each, by sums; a block of which some group would
+// cross max_weight is filed under FC_DEFER for k_fold_b), those of 65..128
+// (a wave each), then chunks of the small classes (one group of 64/G blocks
+// per wave).  LDS: gw windows of 32-bit voxel sums (4 KB each, the launch's
+// dynamic LDS; the wave tier's strips share it) and the pass's scans.  At most 64
+// VGPRs per thread let eight workgroups share a CU.  Workgroup q takes items
+// q, q + grid, ...; with a grid larger than what fits on the chip at once,
+// the dispatcher starts a new workgroup whenever one finishes, which
+// balances the uneven items.
 template <class State>
-__global__ void __launch_bounds__(256, 8) k_fold_a(Dev D) {
-  __shared__ unsigned long long rin[kCountTileS];
-  __shared__ unsigned long long rout[kCountTileS];
-  __shared__ uint32_t cnt[kVoxelsPerBlock];
-  __shared__ uint16_t start[kVoxelsPerBlock + 1];  // (offsets within a tile of kCountTileS records)
-  __shared__ uint32_t sh[kScanSh];
+__global__ void __launch_bounds__(256, 8) k_fold_a(Dev D, int gw) {
+  extern __shared__ uint2 fold_a_acc[];  // gw * 512 32-bit sums
+  __shared__ PassScans ps;
+  __shared__ int32_t bnd[kMaxScansPerBatch];
+  __shared__ int flag;
   unsigned long long uniq = 0;
+  stage_scans(D, ps);
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG]);
   const int n_wgl = ld_acq(&D.ctr->fold_n[FC_WGL]);
   const int n64 = ld_acq(&D.ctr->fold_n[FC_64]);
@@ -1930,30 +2261,35 @@ __global__ void __launch_bounds__(256, 8) k_fold_a(Dev D) {
   const int total = n_wg + cl + c64 + c32 + c16;
   const int4* dwg = D.fold_desc + D.fold_base[FC_WG];
   const int4* dwgl = D.fold_desc + D.fold_base[FC_WGL];
-  static_assert(kCountTileS >= 4 * kWgSplit, "a wave's strip of kWgSplit records in rin");
+  static_assert(kVoxelsPerBlock * sizeof(uint2) >= 4 * kWgSplit * sizeof(unsigned long long),
+                "the wave strips fit one window of sums");
+  static_assert(kCountTileS <= kFixSum32, "the workgroup tier's sums fit 32 bits");
+  __syncthreads();
   for (int it = blockIdx.x; it < total; it += gridDim.x) {
     if (it < n_wg) {
-      fold_count_block<State, kCountTileS, 256, uint16_t>(D, dwg[it], rin, rout, cnt, start, sh, uniq);
+      const int4 d = dwg[it];
+      if (fold_sum_block<State, 256, false>(D, d, fold_a_acc, gw, ps, bnd, &flag, uniq) && threadIdx.x == 0)
+        D.fold_desc[D.fold_base[FC_DEFER] + atomicAdd(&D.ctr->fold_n[FC_DEFER], 1)] = d;
     } else if (it < n_wg + cl) {
       const int q = (it - n_wg) * kl + wave;
       if (q < n_wgl) {
         const int4 d = dwgl[q];
-        unsigned long long* strip = rin + wave * kWgSplit;
-        fold_wave_block<State, kWgSplit / 64>(D, d, strip, uniq);
+        unsigned long long* strip = reinterpret_cast<unsigned long long*>(fold_a_acc) + wave * kWgSplit;
+        fold_wave_block<State, kWgSplit / 64>(D, d, strip, ps, uniq);
       }
-      __syncthreads();  // the strips live in rin, which the next item may use
+      __syncthreads();  // the strips live in the sums' LDS, which the next item may use
     } else {
       int c = it - n_wg - cl;
       if (c < c64) {
         const int q = c * k64 + wave;
-        fold_small_range<State, 64>(D, q, min(q + 1, n64), uniq);
+        fold_small_range<State, 64>(D, q, min(q + 1, n64), ps, uniq);
       } else if ((c -= c64) < c32) {
         const int q = c * k32 + wave * 2;
-        fold_small_range<State, 32>(D, q, min(q + 2, n32), uniq);
+        fold_small_range<State, 32>(D, q, min(q + 2, n32), ps, uniq);
       } else {
         c -= c32;
         const int q = c * k16 + wave * 4;
-        fold_small_range<State, 16>(D, q, min(q + 4, n16), uniq);
+        fold_small_range<State, 16>(D, q, min(q + 4, n16), ps, uniq);
       }
     }
   }
@@ -1961,18 +2297,22 @@ __global__ void __launch_bounds__(256, 8) k_fold_a(Dev D) {
   write_stat_row(D, SR_FOLD_A, row);
 }
 
-// Fold, second launch: blocks of kCountTileS+1..kCountTile records, and the
-// larger ones (folded in voxel ranges, or in point rounds when a voxel holds
-// more than a range); one 64 KB LDS buffer serves as the counting tile pair.
-// 1024 threads; eight waves per SIMD (two workgroups, and their 2 x 68 KB
-// of LDS, per CU) caps the kernel at 64 VGPRs.
+// Fold, second launch: blocks of kCountTileS+1..kCountTile records and the
+// larger ones, by sums (a 1024-thread workgroup each, the 64 KB buffer as
+// eight windows of voxel sums), and the blocks k_fold_a deferred.  A block of
+// which some group would cross max_weight takes the ordered fold, with the
+// buffer as the counting tile pair: a counting sort by voxel and rank
+// placement by point, in voxel ranges that fit the tile for the larger
+// blocks, or in point rounds when a voxel holds more than a range.
+// Eight waves per SIMD (two workgroups, and their 2 x 68 KB of LDS, per CU)
+// cap the kernel at 64 VGPRs.
 // Items are handed out one at a time from a pass counter, largest first
-// (the ranged blocks, then the kCountTile class): blocks differ in cost by an
-// order of magnitude, and a fixed stride over them left some workgroups with
-// several of the largest while the rest had finished.  A pass without ranged
-// blocks (a single scan's: fewer items than workgroups) folds block w of the
-// kCountTile class in workgroup w and touches the counter only for items
-// beyond the grid.
+// (the ranged blocks, then the kCountTile class, then the deferred ones):
+// blocks differ in cost by an order of magnitude, and a fixed stride over
+// them left some workgroups with several of the largest while the rest had
+// finished.  A pass without ranged blocks (a single scan's: fewer items than
+// workgroups) folds item w in workgroup w and touches the counter only for
+// items beyond the grid.
 constexpr int kFoldBThreads = 1024;
 template <class State>
 __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
@@ -1982,35 +2322,48 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
   __shared__ uint32_t start[kVoxelsPerBlock + 1];
   __shared__ uint32_t sh[kScanSh];
   __shared__ uint32_t rstart[kMaxRanges + 1];
-  __shared__ int next_item;
+  __shared__ PassScans ps;
+  __shared__ int32_t bnd[kMaxScansPerBatch];
+  __shared__ int next_item, flag;
+  constexpr int kSumWindows = (int)(kFoldCap * sizeof(unsigned long long) / (kVoxelsPerBlock * sizeof(ulonglong2)));
   unsigned long long uniq = 0;
+  stage_scans(D, ps);
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG2]);
   const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
+  const int n_def = ld_acq(&D.ctr->fold_n[FC_DEFER]);
   const int4* desc = D.fold_desc + D.fold_base[FC_WG2];
   const int4* hdesc = D.fold_desc + D.fold_base[FC_HUGE];
+  const int4* ddesc = D.fold_desc + D.fold_base[FC_DEFER];
+  __syncthreads();
+  auto fold_item = [&](int k) {
+    if (k < n_huge) {
+      const int4 d = hdesc[k];
+      if (fold_sum_block<State, kFoldBThreads, true>(D, d, buf, kSumWindows, ps, bnd, &flag, uniq) &&
+          !fold_ranged_block<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, rstart, sh, ps, uniq))
+        fold_point_rounds<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, sh, ps, uniq);
+    } else if (k < n_huge + n_wg) {
+      const int4 d = desc[k - n_huge];
+      if (fold_sum_block<State, kFoldBThreads, true>(D, d, buf, kSumWindows, ps, bnd, &flag, uniq))
+        fold_count_block<State, kCountTile, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, sh, ps, uniq);
+    } else {
+      fold_count_block<State, kCountTile, kFoldBThreads>(D, ddesc[k - n_huge - n_wg], buf, buf + kCountTile, cnt,
+                                                          start, sh, ps, uniq);
+    }
+  };
+  const int total = n_huge + n_wg + n_def;
   int q0 = 0;  // the first item taken from the counter
   if (n_huge == 0) {
-    if ((int)blockIdx.x < n_wg)
-      fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[blockIdx.x], buf, buf + kCountTile, cnt, start, sh, uniq);
+    if ((int)blockIdx.x < total) fold_item((int)blockIdx.x);
     q0 = (int)gridDim.x;
   }
-  // items in order: the ranged blocks, then the kCountTile class; with ranged
-  // blocks present every item comes from the counter
-  const int total = n_huge + n_wg;
+  // with ranged blocks present every item comes from the counter
   while (q0 < total) {
     if (threadIdx.x == 0) next_item = q0 + atomicAdd(&D.ctr->fold_next, 1);
     __syncthreads();
     const int k = next_item;
     __syncthreads();
     if (k >= total) break;
-    if (k < n_huge) {
-      const int4 d = hdesc[k];
-      if (!fold_ranged_block<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, rstart, sh, uniq))
-        fold_point_rounds<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, sh, uniq);
-    } else {
-      fold_count_block<State, kCountTile, kFoldBThreads>(D, desc[k - n_huge], buf, buf + kCountTile, cnt, start, sh,
-                                                          uniq);
-    }
+    fold_item(k);
   }
   const uint32_t row[1] = {(uint32_t)uniq};
   write_stat_row(D, SR_FOLD_B, row);

@@ -457,7 +457,11 @@ int stage_descs(tsdf_engine* E, const PassHdr& hdr, const ScanDev* scans, int ns
 // of D, whose pair rows and records have `stride` slots per point.  inl: the
 // pass's descriptors as kernel arguments, or nscans = -1 when they were
 // staged.
-int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int tid, const PassInline& inl) {
+// gw: windows of voxel sums k_fold_a keeps in LDS (groups of one block
+// folded per turn over its records): the most scans one map has in the pass,
+// at most kFoldAWindows.
+constexpr int kFoldAWindows = 4;
+int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int tid, const PassInline& inl, int gw) {
   hipStream_t st = E->stream;
   const tsdf_launch_config& L = E->launch;
   CHK(timing_mark(E, tid, 0));
@@ -487,7 +491,8 @@ int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int ti
   constexpr int kFoldBGrid = 512;
   with_state(E->state_format, [&](auto s) {
     using State = decltype(s);
-    hipLaunchKernelGGL(k_fold_a<State>, dim3(fg), dim3(256), 0, st, D);
+    gw = std::min(std::max(gw, 1), kFoldAWindows);
+    hipLaunchKernelGGL(k_fold_a<State>, dim3(fg), dim3(256), (size_t)gw * kVoxelsPerBlock * sizeof(uint2), st, D, gw);
     hipLaunchKernelGGL(k_fold_b<State>, dim3(kFoldBGrid), dim3(kFoldBThreads), 0, st, D);
     return 0;
   });
@@ -576,7 +581,14 @@ int run_batch(tsdf_engine* E, tsdf_map* const* maps, int nmaps, const tsdf_scan_
   }
   int tid;
   CHK(timing_begin(E, 0, &tid));
-  CHK(enqueue_pass(E, E->view(), P, E->stride_cap, tid, inl));
+  // the most scans of one map in the pass
+  int gw = 1;
+  for (int i = 0; i < nscans; ++i) {
+    int same = 0;
+    for (int j = 0; j < nscans; ++j) same += sd[j].map == sd[i].map ? 1 : 0;
+    gw = std::max(gw, same);
+  }
+  CHK(enqueue_pass(E, E->view(), P, E->stride_cap, tid, inl, gw));
   timing_end(E, tid);
   E->last_points = P;
   return TSDF_OK;
@@ -1140,7 +1152,7 @@ int tsdf_integrate_noalloc(tsdf_map* m, const float* points_dev, int64_t n, cons
   for (int k = 0; k < 3; ++k) inl.scans[0].s[k] = sensor[k];
   int tid;
   CHK(timing_begin(E, 0, &tid));
-  CHK(enqueue_pass(E, D, n, stride, tid, inl));
+  CHK(enqueue_pass(E, D, n, stride, tid, inl, 1));
   timing_end(E, tid);
   E->last_points = n;
   return TSDF_OK;
@@ -1330,7 +1342,7 @@ static int graph_capture(tsdf_batch_graph* g) {
   HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
   PassInline staged{};
   staged.nscans = -1;  // a replay's descriptors are copied in before each launch
-  int r = enqueue_pass(E, E->view(), g->max_points, E->stride_cap, -1, staged);
+  int r = enqueue_pass(E, E->view(), g->max_points, E->stride_cap, -1, staged, g->max_scans);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(E->stream, &graph);
   E->timing = t;
