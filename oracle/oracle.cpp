@@ -22,7 +22,10 @@
 //   D3 no 1000-entry cap on GC of chained entries (H:853-858);
 //   D4 blocks of every updated voxel are allocated (H:620/629 index
 //      blocks[-1] when the block DDA missed a voxel's block);
-//   D5 deterministic per-voxel fold order (H:643-664 use lock order);
+//   D5 deterministic per-voxel fold order (H:643-664 use lock order): here
+//      ascending point index; the engine sums each scan's updates of a voxel
+//      exactly, which is this order's value in exact arithmetic until the
+//      voxel reaches max_weight, and folds in this order from there;
 //   D6 each DDA emits at most MAX_*_TRAVERSED cells (H:1311, H:1315) and
 //      invalid points (non-finite, zero range, |coord| > 2^16 voxels) are
 //      skipped (H:168 never terminates on NaN);

@@ -13,15 +13,16 @@
 //   k_emit           per point: second voxel-DDA walk writing each update
 //                    record (voxel, point, sdf) to its block's contiguous
 //                    record segment at the offset reserved in k_bin
-//   k_fold_a         blocks of 129..1024 records (a workgroup each: counting
-//                    sort by voxel in LDS, rank placement by point, fold),
-//                    blocks of 65..128 (a wave each) and of <= 64 / 32 / 16
-//                    records, 1 / 2 / 4 per wave (bitonic sort across lanes;
-//                    each voxel's first record folds the voxel's updates)
-//   k_fold_b         blocks of 1025..4096 records (counting sort), then
-//                    larger ones in voxel ranges that fit the counting tile
-// Every voxel's state is read once and written once per pass, its updates
-// folded in point order.  Pass statistics are per-workgroup rows, summed on
+//   k_fold_a         blocks of 129..1024 records (a workgroup each: per-voxel
+//                    sums in LDS), blocks of 65..128 (a wave each) and of
+//                    <= 64 / 32 / 16 records, 1 / 2 / 4 per wave (bitonic
+//                    sort across lanes; each voxel's first record sums the
+//                    voxel's updates)
+//   k_fold_b         larger blocks by per-voxel sums, and the ordered fold of
+//                    blocks with a group that crosses max_weight
+// Every voxel's state is read once and written once per pass; its updates of
+// each scan are applied as one weighted mean (see "the fold of one voxel's
+// updates in a pass").  Pass statistics are per-workgroup rows, summed on
 // demand (k_pass_stats).
 //
 // publish: k_select (radius test per entry) -> k_pub_count (wave per block)
