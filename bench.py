@@ -230,7 +230,7 @@ def main():
     kernel_stages = ["traverse", "assign", "scan", "scatter", "fold"]
     dom = max(kernel_stages, key=lambda k: stage_ms[k])
     # stage -> the launches between its two timing events ("*": a group of launches)
-    dom_kernel = {"traverse": "k_bin", "assign": "k_bin", "scan": "k_pass_scan+k_seg_apply", "scatter": "k_emit",
+    dom_kernel = {"traverse": "k_bin", "assign": "k_bin", "scan": "k_seg", "scatter": "k_emit",
                   "fold": "k_fold_a+k_fold_b"}[dom]
     achieved = alg_bytes / (stage_ms[dom] * 1e-3)
     traffic = None

@@ -20,7 +20,7 @@ extern "C" {
 /* Atomic patterns, by lane:
  * SCATTER_RET64: 64-bit add with return, every lane on its own line of a
  *   128 MiB table, lines hashed per lane and step (k_bin's per-block record
- *   reservations, k_pass_scan's heap pops);
+ *   reservations, k_seg's heap pops);
  * SCATTER_NORET32: 32-bit add, result unused, same addressing;
  * COALESCED_RET32: 32-bit add with return, a wave's 64 lanes on 64
  *   consecutive words (256 bytes), waves on hashed places of the table;

@@ -28,9 +28,9 @@ enum Stat {
 // its own row of counts (no atomics, and nothing to clear between passes),
 // and the rows are summed on demand.  Each kernel has a region of rows, one
 // per workgroup of its largest grid.
-enum StatRegion { SR_BIN = 0, SR_SCAN, SR_SEG, SR_FOLD_A, SR_FOLD_B, SR_COUNT };
+enum StatRegion { SR_BIN = 0, SR_SCAN, SR_FOLD_A, SR_FOLD_B, SR_COUNT };
 constexpr int kStatCols = 8;  // counts per row
-constexpr int kStatRows[SR_COUNT] = {65536, 256, 256, 65536, 512};
+constexpr int kStatRows[SR_COUNT] = {65536, 256, 65536, 512};
 __host__ __device__ constexpr int stat_region_base(int r) {
   int b = 0;
   for (int k = 0; k < r; ++k) b += kStatRows[k];
@@ -71,6 +71,8 @@ struct EngCtr {
   int32_t new_n;      // entries inserted in this pass, waiting for heap blocks
   int32_t fold_next;  // large-block fold items handed out beyond the first per workgroup
   int32_t bin_next;   // k_bin ray chunks handed out from the work queue beyond the first per workgroup
+  int32_t pass_epoch; // passes so far (tags the look-back states of the pass scan)
+  int32_t pad_e;
   int32_t fold_n[8];  // touched blocks per fold size class
   unsigned long long stats[ST_COUNT];
 };
@@ -115,10 +117,11 @@ struct Dev {
   uint32_t* touched;         // [entries]
   uint8_t* touched_map;      // [entries] map slot of each touched block
   uint32_t* tile_sums;       // [entries / 1024 + 2]
+  unsigned long long* tile_state;  // [entries / 1024 + 2] look-back states of the pass scan (k_seg)
   uint32_t* fail_list;       // [entries] inserted entries without a heap block (removed after binning)
   uint32_t* new_list;        // [entries] entries inserted in this pass (heap blocks assigned after binning)
   uint8_t* new_map;          // [entries] their map slots
-  int4* fold_desc;           // per size class c, from fold_base[c]: {record base, records, heap block, map}
+  int4* fold_desc;           // per size class c, from fold_base[c]: {record base, records, entry, map}
   unsigned long long* part;  // [ST_COUNT][kPartSlots] statistics partial sums (publish and gc)
   uint32_t* srows;           // [kStatRowsTotal][kStatCols] pass statistics, a row per workgroup
   int64_t fold_cap;

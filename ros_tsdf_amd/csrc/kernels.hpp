@@ -6,10 +6,10 @@
 //                    (lock-free) once per workgroup; each run of voxels in
 //                    one block gets its record offset in that block; new
 //                    entries join the pass's new-entry list
-//   k_pass_scan      per-tile totals of the touched blocks' record counts;
-//                    heap blocks for the new entries (one pop per wave)
-//   k_seg_apply      record segment offset and 16-byte fold descriptor of
-//                    every touched block, filed by size class
+//   k_seg            heap blocks for the new entries (one pop per wave);
+//                    record segment offset and 16-byte fold descriptor of
+//                    every touched block, filed by size class (a scan over
+//                    the touched list with decoupled look-back)
 //   k_emit           per point: second voxel-DDA walk writing each update
 //                    record (voxel, point, sdf) to its block's contiguous
 //                    record segment at the offset reserved in k_bin
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(kPassStatsThreads) k_pass_stats(Dev D, StatGri
   const int st = blockIdx.x;
   if (st >= ST_COUNT || !(mask & (1u << st))) return;
   // (region, column) pairs that hold the statistic
-  int reg[2] = {-1, -1}, col[2] = {0, 0};
+  int reg[4] = {-1, -1, -1, -1}, col[4] = {0, 0, 0, 0};
   switch (st) {
     case ST_VALID: reg[0] = SR_BIN; col[0] = 0; break;
     case ST_SKIPPED: reg[0] = SR_BIN; col[0] = 1; break;
@@ -226,7 +226,8 @@ __global__ void __launch_bounds__(kPassStatsThreads) k_pass_stats(Dev D, StatGri
     case ST_TRUNC_VOX: reg[0] = SR_BIN; col[0] = 4; break;
     case ST_TRUNC_BLK: reg[0] = SR_BIN; col[0] = 5; break;
     case ST_INSERT_FAIL: reg[0] = SR_BIN; col[0] = 6; break;
-    case ST_UPD_DROPPED: reg[0] = SR_BIN; col[0] = 7; reg[1] = SR_SEG; col[1] = 0; break;
+    case ST_UPD_DROPPED: reg[0] = SR_BIN; col[0] = 7; reg[1] = SR_FOLD_A; col[1] = 1; reg[2] = SR_FOLD_B; col[2] = 1;
+      reg[3] = SR_SCAN; col[3] = 3; break;
     case ST_NEW_BLOCKS: reg[0] = SR_SCAN; col[0] = 0; break;
     case ST_HEAP_FAIL: reg[0] = SR_SCAN; col[0] = 1; break;
     case ST_QUOTA_FAIL: reg[0] = SR_SCAN; col[0] = 2; break;
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(kPassStatsThreads) k_pass_stats(Dev D, StatGri
     default: break;  // ST_INDEX_DROPPED: no kernel counts it (valid points never produce it)
   }
   unsigned long long v = 0;
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < 4; ++q) {
     if (reg[q] < 0) continue;
     const uint32_t* r = D.srows + (int64_t)stat_region_base(reg[q]) * kStatCols + col[q];
     const int n = g.n[reg[q]];
@@ -629,10 +630,14 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched 
   const PassHdr hdr = inl.nscans >= 0 ? inl.hdr : *D.hdr;
   const int P = hdr.total;
   if (blockIdx.x == 0) {
-    // the previous pass's fold is over: clear its counters; store the
-    // inline descriptors for k_emit
+    // the previous pass's fold is over: clear its counters, start a new
+    // pass epoch; store the inline descriptors for k_emit
     const int t = threadIdx.x;
-    if (t == 0) D.ctr->fold_next = 0;
+    if (t == 0) {
+      D.ctr->fold_next = 0;
+      D.ctr->fail_n = 0;  // (its last reader, k_fold_b, is over)
+      D.ctr->pass_epoch++;
+    }
     if (t < FC_COUNT) D.ctr->fold_n[t] = 0;
     if (inl.nscans >= 0) {
       if (t == 0) *const_cast<PassHdr*>(D.hdr) = inl.hdr;
@@ -645,7 +650,7 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched 
   uint32_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // a new entry joins the workgroup's list, which goes to the pass's
   // new-entry list with one atomic at the end of the workgroup's points
-  // (heap blocks are assigned by k_pass_scan), or joins the pass's list at
+  // (heap blocks are assigned by k_seg), or joins the pass's list at
   // once when the workgroup's list is full
   auto note_new = [&](int e, int ms) {
     const int at = atomicAdd(&n_new, 1);
@@ -1061,31 +1066,47 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int32_t* n_ptr, Count 
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_sums[ntiles];
 }
 
-// First kernel after binning: per-tile record totals of the touched blocks
-// (the tile sums of an exclusive scan over the touched list), and heap blocks for the pass's
-// new entries, 256 per workgroup and chunk (the chunks are taken from the
-// last workgroup down, so the tile workgroups get them last).  Heap blocks
-// come off the free stack with one pop, and one per-map count, per wave.
-__global__ void __launch_bounds__(256) k_pass_scan(Dev D) {
+// Look-back state of one tile of the touched list in the pass scan: [pass
+// epoch (30 bits) | kind (2 bits) | records (32 bits)], one 64-bit word, so
+// a state and its value are published together.  kind 1: the tile's own
+// records; kind 2: the records of the tiles up to it.  A word of another
+// epoch is a previous pass's: not published yet in this one.
+constexpr uint32_t kTileOwn = 1, kTileUpTo = 2;
+constexpr int kTileSpin = 1 << 22;  // bound on waiting for a predecessor (never reached: see k_seg)
+__device__ __forceinline__ unsigned long long tile_word(uint32_t epoch, uint32_t kind, uint32_t v) {
+  return ((unsigned long long)(((epoch & 0x3FFFFFFFu) << 2) | kind) << 32) | v;
+}
+
+// The pass scan, one launch after binning:
+//  * heap blocks for the pass's new entries, 256 per workgroup and chunk
+//    (taken from the last workgroup down, so the tile workgroups get them
+//    last); off the free stack with one pop, and one per-map count, per wave;
+//  * record segments and fold descriptors of the touched blocks, per tile of
+//    the touched list: the tile publishes its record total, then sums the
+//    records of the tiles before it from its predecessors' published states
+//    (decoupled look-back: a predecessor's "up to" total ends the walk, its
+//    own total adds and the walk goes on) and publishes its own "up to"
+//    total.  Each block's segment offset follows by a workgroup scan; its
+//    16-byte descriptor {record base, records, entry, map} is filed under its
+//    size class with one counter atomic per class and workgroup, and its pass
+//    counter is reset.  The descriptor holds the entry, not its heap block:
+//    the heap blocks of new entries are being assigned beside it, and the
+//    fold reads them.
+// A workgroup waits only for tiles of lower index, which lower workgroups
+// take in the same or an earlier turn, and the grid (at most 256 workgroups
+// of 256 threads) is resident at once: every wait ends.  The bound on it is
+// a guard only; a tile that reached it would count its records as dropped.
+__global__ void __launch_bounds__(256) k_seg(Dev D) {
   __shared__ uint32_t sh[kScanSh];
+  __shared__ int lcnt[FC_COUNT], gbase[FC_COUNT];
+  __shared__ uint32_t carry_sh;
   const int n = ld_acq(&D.ctr->touched_n);
-  const int nn = ld_acq(&D.ctr->new_n);  // (read with the other count: one round trip)
-  const int ntiles = (n + kScanTile - 1) / kScanTile;
-  const CountTouchedRecords cnt{D.touched, D.blk_cnt};
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    uint32_t v = 0;
-#pragma unroll
-    for (int k = 0; k < kScanTile / 256; ++k) {
-      const int i = t * kScanTile + k * 256 + threadIdx.x;
-      if (i < n) v += cnt(i);
-    }
-    uint32_t tot;
-    block_scan_excl(v, sh, tot);
-    if (threadIdx.x == 0) D.tile_sums[t] = tot;
-  }
+  const int nn = ld_acq(&D.ctr->new_n);  // (read with the other counts: one round trip)
+  const uint32_t epoch = (uint32_t)ld_acq(&D.ctr->pass_epoch);
   // heap failures include quota failures, which are also counted apart
-  // (columns of the SR_SCAN rows: ST_NEW_BLOCKS, ST_HEAP_FAIL, ST_QUOTA_FAIL)
-  uint32_t st[3] = {0, 0, 0};
+  // (columns of the SR_SCAN rows: ST_NEW_BLOCKS, ST_HEAP_FAIL, ST_QUOTA_FAIL,
+  // and records dropped by a tile whose wait reached its bound)
+  uint32_t st[4] = {0, 0, 0, 0};
   const int nchunks = (nn + 255) / 256;
   for (int c = gridDim.x - 1 - blockIdx.x; c < nchunks; c += gridDim.x) {
     const int x = c * 256 + threadIdx.x;
@@ -1094,6 +1115,80 @@ __global__ void __launch_bounds__(256) k_pass_scan(Dev D) {
     st[0] += r == kAssignOk ? 1 : 0;
     st[1] += (r == kAssignHeapFail || r == kAssignQuotaFail) ? 1 : 0;
     st[2] += r == kAssignQuotaFail ? 1 : 0;
+  }
+  const int ntiles = (n + kScanTile - 1) / kScanTile;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    // the tile's loads first: touched entries, then their counters and maps
+    const int i0 = t * kScanTile + threadIdx.x * 4;
+    uint32_t e4[4], cn[4];
+    uint8_t mp4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) e4[k] = i0 + k < n ? D.touched[i0 + k] : 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      cn[k] = 0;
+      mp4[k] = 0;
+      if (i0 + k < n) {
+        cn[k] = (uint32_t)(D.blk_cnt[e4[k]] >> 32);
+        mp4[k] = D.touched_map[i0 + k];
+      }
+    }
+    if (threadIdx.x < FC_COUNT) lcnt[threadIdx.x] = 0;
+    uint32_t tot;
+    uint32_t ex = block_scan_excl(cn[0] + cn[1] + cn[2] + cn[3], sh, tot);
+    if (threadIdx.x == 0) {
+      uint32_t before = 0;
+      if (t == 0) {
+        __hip_atomic_store(&D.tile_state[0], tile_word(epoch, kTileUpTo, tot), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_store(&D.tile_state[t], tile_word(epoch, kTileOwn, tot), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        int j = t - 1;
+        for (int spins = 0;;) {
+          const unsigned long long w = __hip_atomic_load(&D.tile_state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const uint32_t tag = (uint32_t)(w >> 32);
+          if ((tag >> 2) == (epoch & 0x3FFFFFFFu) && (tag & 3u) != 0) {
+            before += (uint32_t)w;
+            if ((tag & 3u) == kTileUpTo || j == 0) break;
+            --j;
+          } else if (++spins > kTileSpin) {
+            st[3] += tot;
+            before = 0;
+            break;
+          }
+        }
+        __hip_atomic_store(&D.tile_state[t], tile_word(epoch, kTileUpTo, before + tot), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+      carry_sh = before;
+    }
+    __syncthreads();
+    ex += carry_sh;
+    int cls[4], rank[4];
+    int4 dsc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      cls[k] = -1;
+      rank[k] = 0;
+      dsc[k] = make_int4(0, 0, 0, 0);
+      if (i0 + k < n) {
+        D.blk_base[e4[k]] = ex;
+        D.blk_cnt[e4[k]] = 0;
+        dsc[k] = make_int4((int)ex, (int)cn[k], (int)e4[k], (int)mp4[k]);
+        cls[k] = fold_class((int)cn[k]);
+        rank[k] = atomicAdd(&lcnt[cls[k]], 1);
+      }
+      ex += cn[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < FC_COUNT)
+      gbase[threadIdx.x] = lcnt[threadIdx.x] ? atomicAdd(&D.ctr->fold_n[threadIdx.x], lcnt[threadIdx.x]) : 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (cls[k] >= 0) D.fold_desc[D.fold_base[cls[k]] + gbase[cls[k]] + rank[k]] = dsc[k];
+    __syncthreads();
   }
   write_stat_row(D, SR_SCAN, st);
 }
@@ -1151,10 +1246,9 @@ __global__ void __launch_bounds__(256) k_emit(Dev D) {
   const int P = hdr.total;
   const int64_t stride = D.pcap;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // k_pass_scan and k_seg_apply, their last readers, are over
+    // k_seg, their last reader, is over
     D.ctr->touched_n = 0;
     D.ctr->new_n = 0;
-    D.ctr->fail_n = 0;
     D.ctr->bin_next = 0;  // (k_bin's work queue)
   }
   // run rows that exist in the pair arrays (they hold hdr.stride rows)
@@ -1392,6 +1486,19 @@ __device__ __forceinline__ int scan_in_lds(const PassScans& ps, int p, int from)
   return lo;
 }
 
+// A fold kernel's counts for the pass statistics: voxels folded, and
+// records of blocks whose entry got no heap block (dropped).
+struct FoldTally {
+  unsigned long long uniq = 0;
+  uint32_t dropped = 0;
+};
+// A fold descriptor with its entry (d.z) replaced by the entry's heap block,
+// -1 when the entry got none.
+__device__ __forceinline__ int4 resolve_desc(const Dev& D, int4 d) {
+  d.z = D.vals[d.z];
+  return d;
+}
+
 // The ordered fold of one voxel's updates, fed in ascending point index
 // (possibly over several calls, as the point rounds do).
 template <class State>
@@ -1495,11 +1602,12 @@ __device__ __forceinline__ int group_of(int p, int G, int32_t b1, int32_t b2, in
 // order by the voxel's thread.  A block of at most 4 NT records keeps them
 // in registers across the turns; a larger one reads them again per turn.
 // Returns true, with nothing stored, when some group would cross maxW: the
-// block then takes an ordered fold.  bnd: LDS for the map's group starts
-// (kMaxScansPerBatch entries); flag: an LDS word.
+// block then takes an ordered fold.  d.z is the heap block (resolve_desc).
+// bnd: LDS for the map's group starts (kMaxScansPerBatch entries); flag: an
+// LDS word.
 template <class State, int NT, bool kWide>
 __device__ bool fold_sum_block(const Dev& D, const int4 d, void* acc, int gw, const PassScans& ps,
-                               int32_t* bnd, int* flag, unsigned long long& uniq) {
+                               int32_t* bnd, int* flag, FoldTally& tally) {
   using Pair = typename std::conditional<kWide, ulonglong2, uint2>::type;
   Pair* const accp = reinterpret_cast<Pair*>(acc);
   constexpr int kVpt = NT >= kVoxelsPerBlock ? 1 : kVoxelsPerBlock / NT;  // voxels per thread
@@ -1522,6 +1630,10 @@ __device__ bool fold_sum_block(const Dev& D, const int4 d, void* acc, int gw, co
   unsigned long long r[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) r[u] = held && t + u * NT < c ? sg[t + u * NT] : 0ull;
+  if (d.z < 0) {  // the entry got no heap block: the block's records are dropped
+    if (t == 0) tally.dropped += (uint32_t)c;
+    return false;
+  }
   for (int wb = 0;; wb += gw) {
     const int nw = min(gw, G - wb);
     for (int x = t; x < nw * kVoxelsPerBlock; x += NT) accp[x] = Pair{0, 0};
@@ -1590,7 +1702,7 @@ __device__ bool fold_sum_block(const Dev& D, const int4 d, void* acc, int gw, co
     const int l = t + NT * u;
     if (l < kVoxelsPerBlock && seen[u]) {
       State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, S[u], W[u]);
-      uniq++;
+      tally.uniq++;
     }
   }
   return false;
@@ -1604,87 +1716,6 @@ __device__ __forceinline__ void fold_run(OrderedFold<State>& of, const unsigned 
 }
 
 
-// Record segments and fold descriptors of the touched blocks.  Per tile of
-// the touched list (tile totals from k_pass_scan): the tile's carry is the
-// sum of the preceding totals (summed by the workgroup itself, so no
-// separate top-level scan is launched), each block's segment offset follows
-// by a workgroup scan, and its 16-byte descriptor {record base, records,
-// heap block, map} is filed under its size class with one counter atomic
-// per class and workgroup.  The pass counters of the blocks are reset for
-// the next pass, and entries that got no heap block leave the table.
-__global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
-  __shared__ unsigned long long sh64[kScanSh];
-  __shared__ int lcnt[FC_COUNT], gbase[FC_COUNT];
-  const int n = ld_acq(&D.ctr->touched_n);
-  const int nf = ld_acq(&D.ctr->fail_n);  // (read with the other count: one round trip)
-  const int ntiles = (n + kScanTile - 1) / kScanTile;
-  unsigned long long dropped = 0;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    // the tile's loads first (touched entries, then their counters, heap
-    // blocks and maps, all independent of each other), the preceding tile
-    // totals beside them
-    const int i0 = t * kScanTile + threadIdx.x * 4;
-    uint32_t e4[4], cn[4];
-    int sl4[4];
-    uint8_t mp4[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) e4[k] = i0 + k < n ? D.touched[i0 + k] : 0u;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      cn[k] = 0;
-      sl4[k] = -1;
-      mp4[k] = 0;
-      if (i0 + k < n) {
-        cn[k] = (uint32_t)(D.blk_cnt[e4[k]] >> 32);
-        sl4[k] = D.vals[e4[k]];
-        mp4[k] = D.touched_map[i0 + k];
-      }
-    }
-    uint32_t c = 0;
-    for (int k = threadIdx.x; k < t; k += blockDim.x) c += D.tile_sums[k];
-    if (threadIdx.x < FC_COUNT) lcnt[threadIdx.x] = 0;
-    const uint32_t v = cn[0] + cn[1] + cn[2] + cn[3];
-    // one scan yields both: preceding tiles' total in the high half, this
-    // thread's record offset in the low half (a tile's records fit in 32 bits)
-    unsigned long long tot;
-    const unsigned long long exv = block_scan_excl64(((unsigned long long)c << 32) | v, sh64, tot);
-    const uint32_t carry = (uint32_t)(tot >> 32);
-    uint32_t ex = (uint32_t)exv + carry;
-    int cls[4], rank[4];
-    int4 dsc[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      cls[k] = -1;
-      rank[k] = 0;
-      dsc[k] = make_int4(0, 0, 0, 0);
-      if (i0 + k < n) {
-        D.blk_base[e4[k]] = ex;
-        D.blk_cnt[e4[k]] = 0;
-        if (sl4[k] < 0) {
-          dropped += cn[k];
-        } else {
-          dsc[k] = make_int4((int)ex, (int)cn[k], sl4[k], (int)mp4[k]);
-          cls[k] = fold_class((int)cn[k]);
-          rank[k] = atomicAdd(&lcnt[cls[k]], 1);
-        }
-      }
-      ex += cn[k];
-    }
-    __syncthreads();
-    if (threadIdx.x < FC_COUNT)
-      gbase[threadIdx.x] = lcnt[threadIdx.x] ? atomicAdd(&D.ctr->fold_n[threadIdx.x], lcnt[threadIdx.x]) : 0;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (cls[k] >= 0) D.fold_desc[D.fold_base[cls[k]] + gbase[cls[k]] + rank[k]] = dsc[k];
-    __syncthreads();
-  }
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nf; q += gridDim.x * blockDim.x)
-    D.keys[D.fail_list[q]] = kEmptyKey;
-  const uint32_t row[1] = {(uint32_t)dropped};
-  write_stat_row(D, SR_SEG, row);
-}
-
 // Blocks of at most G records, 64/G blocks per wave: each group of G lanes
 // holds one block's records, bitonic-sorts them with shuffles, and every
 // run of one voxel is folded by its first lane pulling the run's weight
@@ -1692,7 +1723,7 @@ __global__ void __launch_bounds__(256) k_seg_apply(Dev D) {
 // One wave folds the class-G descriptors [q_begin, q_end), 64/G at a time.
 template <class State, int G>
 __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int q_end, const PassScans& ps,
-                                                 unsigned long long& uniq) {
+                                                 FoldTally& tally) {
   constexpr int kPer = 64 / G;
   constexpr int kCls = G == 16 ? FC_16 : G == 32 ? FC_32 : FC_64;
   const int lane = lane_id();
@@ -1703,7 +1734,7 @@ __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int 
   for (int q0 = q_begin; q0 < n_desc; q0 += kPer) {
     const int q = q0 + grp;
     int4 d = make_int4(0, 0, 0, 0);
-    if (q < n_desc) d = desc[q];
+    if (q < n_desc) d = resolve_desc(D, desc[q]);
     const int n = d.y;
     unsigned long long v = gl < n ? D.seg[(uint32_t)d.x + gl] : ~0ull;
 #pragma unroll
@@ -1716,7 +1747,10 @@ __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int 
       }
     }
     const unsigned long long prev = prev_lane64(v);
-    const bool head = gl < n && (gl == 0 || rec_voxel(prev) != rec_voxel(v));
+    // (an entry without a heap block: the block's records are dropped)
+    const bool live = d.z >= 0;
+    if (!live && gl == 0) tally.dropped += (uint32_t)n;
+    const bool head = live && gl < n && (gl == 0 || rec_voxel(prev) != rec_voxel(v));
     const unsigned long long heads = __ballot(head);
     const int gend = grp * G + n;  // first lane past this group's records
     const unsigned long long above = lane == 63 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
@@ -1769,7 +1803,7 @@ __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int 
     if (head) {
       of.close();
       State::store(D.heap, vi, cd, of.S, of.W);
-      uniq++;
+      tally.uniq++;
     }
   }
 }
@@ -1782,7 +1816,7 @@ __device__ __forceinline__ void fold_small_range(const Dev& D, int q_begin, int 
 // No workgroup barrier: the wave only waits for its own LDS traffic.
 template <class State, int R>
 __device__ __forceinline__ void fold_wave_block(const Dev& D, const int4 d, unsigned long long* strip,
-                                                const PassScans& ps, unsigned long long& uniq) {
+                                                const PassScans& ps, FoldTally& tally) {
   constexpr int N = 64 * R;
   const int lane = lane_id();
   const int n = d.y;
@@ -1790,6 +1824,10 @@ __device__ __forceinline__ void fold_wave_block(const Dev& D, const int4 d, unsi
   unsigned long long v[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) v[r] = 64 * r + lane < n ? sg[64 * r + lane] : ~0ull;
+  if (d.z < 0) {  // the entry got no heap block: the block's records are dropped
+    if (lane == 0) tally.dropped += (uint32_t)n;
+    return;
+  }
 #pragma unroll
   for (int k = 2; k <= N; k <<= 1) {
 #pragma unroll
@@ -1851,7 +1889,7 @@ __device__ __forceinline__ void fold_wave_block(const Dev& D, const int4 d, unsi
     }
     of.close();
     State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, of.S, of.W);
-    uniq++;
+    tally.uniq++;
   }
 }
 
@@ -1881,7 +1919,7 @@ __device__ __forceinline__ void bitonic_sort_lds(unsigned long long* a, int n) {
 template <class State, int TILE, int NT, class StartT = uint32_t>
 __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, unsigned long long* rin,
                                                  unsigned long long* rout, uint32_t* cnt, StartT* start,
-                                                 uint32_t* sh, const PassScans& ps, unsigned long long& uniq) {
+                                                 uint32_t* sh, const PassScans& ps, FoldTally& tally) {
   static_assert(NT == 256 || NT == 512 || NT == 1024, "workgroup of 256, 512 or 1024 threads");
   constexpr int kPer = TILE / NT;                                         // records per thread
   constexpr int kVpt = NT >= kVoxelsPerBlock ? 1 : kVoxelsPerBlock / NT;  // voxels per thread
@@ -1984,7 +2022,7 @@ __device__ __forceinline__ void fold_count_block(const Dev& D, const int4 d, uns
     fold_run(of, rin, (int)start[l], (int)start[l + 1], maxW, cd, ps);
     of.close();
     State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, of.S, of.W);
-    uniq++;
+    tally.uniq++;
   }
   __syncthreads();
 }
@@ -2007,7 +2045,7 @@ template <class State, int NT>
 __device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, unsigned long long* rin,
                                                   unsigned long long* rout, uint32_t* cnt, uint32_t* start,
                                                   uint32_t* rstart, uint32_t* sh, const PassScans& ps,
-                                                  unsigned long long& uniq) {
+                                                  FoldTally& tally) {
   static_assert(NT >= kVoxelsPerBlock, "a thread per voxel");
   static_assert(2 * kRangeSpan <= kCountTile, "a range fits the counting tile");
   __shared__ int too_big, nranges_sh, maxc_sh;
@@ -2088,7 +2126,7 @@ __device__ __forceinline__ bool fold_ranged_block(const Dev& D, const int4 d, un
       fold_run(of, rin, (int)(ex - base), (int)(start[t + 1] - base), m.maxW, cd, ps);
       of.close();
       State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, of.S, of.W);
-      uniq++;
+      tally.uniq++;
     }
     __syncthreads();
   }
@@ -2115,7 +2153,7 @@ constexpr int kRoundBins = 1 << (kPointIdxBits - kRoundBinShift);  // first-leve
 template <class State, int NT>
 __device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long* rin, unsigned long long* rout,
                                   uint32_t* cnt, uint32_t* start, uint32_t* sh, const PassScans& ps,
-                                  unsigned long long& uniq) {
+                                  FoldTally& tally) {
   static_assert(NT == (1 << kRoundBinShift), "a thread per point of a second-level bin");
   static_assert(NT >= kVoxelsPerBlock, "a thread per voxel");
   static_assert(kRoundBins * sizeof(uint32_t) <= kCountTile * sizeof(unsigned long long), "histogram fits in rout");
@@ -2225,7 +2263,7 @@ __device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long
   if (h) {
     of.close();
     State::store(D.heap, (int64_t)d.z * kVoxelsPerBlock + t, cd, of.S, of.W);
-    uniq++;
+    tally.uniq++;
   }
   __syncthreads();
 }
@@ -2246,7 +2284,7 @@ __global__ void __launch_bounds__(256, 8) k_fold_a(Dev D, int gw) {
   __shared__ PassScans ps;
   __shared__ int32_t bnd[kMaxScansPerBatch];
   __shared__ int flag;
-  unsigned long long uniq = 0;
+  FoldTally tally;
   stage_scans(D, ps);
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG]);
   const int n_wgl = ld_acq(&D.ctr->fold_n[FC_WGL]);
@@ -2268,38 +2306,39 @@ __global__ void __launch_bounds__(256, 8) k_fold_a(Dev D, int gw) {
   __syncthreads();
   for (int it = blockIdx.x; it < total; it += gridDim.x) {
     if (it < n_wg) {
-      const int4 d = dwg[it];
-      if (fold_sum_block<State, 256, false>(D, d, fold_a_acc, gw, ps, bnd, &flag, uniq) && threadIdx.x == 0)
-        D.fold_desc[D.fold_base[FC_DEFER] + atomicAdd(&D.ctr->fold_n[FC_DEFER], 1)] = d;
+      const int4 d0 = dwg[it];
+      if (fold_sum_block<State, 256, false>(D, resolve_desc(D, d0), fold_a_acc, gw, ps, bnd, &flag, tally) &&
+          threadIdx.x == 0)
+        D.fold_desc[D.fold_base[FC_DEFER] + atomicAdd(&D.ctr->fold_n[FC_DEFER], 1)] = d0;
     } else if (it < n_wg + cl) {
       const int q = (it - n_wg) * kl + wave;
       if (q < n_wgl) {
-        const int4 d = dwgl[q];
+        const int4 d = resolve_desc(D, dwgl[q]);
         unsigned long long* strip = reinterpret_cast<unsigned long long*>(fold_a_acc) + wave * kWgSplit;
-        fold_wave_block<State, kWgSplit / 64>(D, d, strip, ps, uniq);
+        fold_wave_block<State, kWgSplit / 64>(D, d, strip, ps, tally);
       }
       __syncthreads();  // the strips live in the sums' LDS, which the next item may use
     } else {
       int c = it - n_wg - cl;
       if (c < c64) {
         const int q = c * k64 + wave;
-        fold_small_range<State, 64>(D, q, min(q + 1, n64), ps, uniq);
+        fold_small_range<State, 64>(D, q, min(q + 1, n64), ps, tally);
       } else if ((c -= c64) < c32) {
         const int q = c * k32 + wave * 2;
-        fold_small_range<State, 32>(D, q, min(q + 2, n32), ps, uniq);
+        fold_small_range<State, 32>(D, q, min(q + 2, n32), ps, tally);
       } else {
         c -= c32;
         const int q = c * k16 + wave * 4;
-        fold_small_range<State, 16>(D, q, min(q + 4, n16), ps, uniq);
+        fold_small_range<State, 16>(D, q, min(q + 4, n16), ps, tally);
       }
     }
   }
-  const uint32_t row[1] = {(uint32_t)uniq};
+  const uint32_t row[2] = {(uint32_t)tally.uniq, tally.dropped};
   write_stat_row(D, SR_FOLD_A, row);
 }
 
-// Fold, second launch: blocks of kCountTileS+1..kCountTile records and the
-// larger ones, by sums (a 1024-thread workgroup each, the 64 KB buffer as
+// Fold, second launch (which also takes the pass's failed inserts out of the
+// table): blocks of kCountTileS+1..kCountTile records and the larger ones, by sums (a 1024-thread workgroup each, the 64 KB buffer as
 // eight windows of voxel sums), and the blocks k_fold_a deferred.  A block of
 // which some group would cross max_weight takes the ordered fold, with the
 // buffer as the counting tile pair: a counting sort by voxel and rank
@@ -2327,7 +2366,7 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
   __shared__ int32_t bnd[kMaxScansPerBatch];
   __shared__ int next_item, flag;
   constexpr int kSumWindows = (int)(kFoldCap * sizeof(unsigned long long) / (kVoxelsPerBlock * sizeof(ulonglong2)));
-  unsigned long long uniq = 0;
+  FoldTally tally;
   stage_scans(D, ps);
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG2]);
   const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
@@ -2338,19 +2377,25 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
   __syncthreads();
   auto fold_item = [&](int k) {
     if (k < n_huge) {
-      const int4 d = hdesc[k];
-      if (fold_sum_block<State, kFoldBThreads, true>(D, d, buf, kSumWindows, ps, bnd, &flag, uniq) &&
-          !fold_ranged_block<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, rstart, sh, ps, uniq))
-        fold_point_rounds<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, sh, ps, uniq);
+      const int4 d = resolve_desc(D, hdesc[k]);
+      if (fold_sum_block<State, kFoldBThreads, true>(D, d, buf, kSumWindows, ps, bnd, &flag, tally) &&
+          !fold_ranged_block<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, rstart, sh, ps, tally))
+        fold_point_rounds<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, sh, ps, tally);
     } else if (k < n_huge + n_wg) {
-      const int4 d = desc[k - n_huge];
-      if (fold_sum_block<State, kFoldBThreads, true>(D, d, buf, kSumWindows, ps, bnd, &flag, uniq))
-        fold_count_block<State, kCountTile, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, sh, ps, uniq);
+      const int4 d = resolve_desc(D, desc[k - n_huge]);
+      if (fold_sum_block<State, kFoldBThreads, true>(D, d, buf, kSumWindows, ps, bnd, &flag, tally))
+        fold_count_block<State, kCountTile, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, sh, ps, tally);
     } else {
-      fold_count_block<State, kCountTile, kFoldBThreads>(D, ddesc[k - n_huge - n_wg], buf, buf + kCountTile, cnt,
-                                                          start, sh, ps, uniq);
+      // (k_fold_a deferred it after it found its heap block)
+      fold_count_block<State, kCountTile, kFoldBThreads>(D, resolve_desc(D, ddesc[k - n_huge - n_wg]), buf,
+                                                          buf + kCountTile, cnt, start, sh, ps, tally);
     }
   };
+  // entries inserted in this pass that got no heap block leave the table
+  // (binning, which may insert keys, is over)
+  const int nf = ld_acq(&D.ctr->fail_n);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nf; q += gridDim.x * blockDim.x)
+    D.keys[D.fail_list[q]] = kEmptyKey;
   const int total = n_huge + n_wg + n_def;
   int q0 = 0;  // the first item taken from the counter
   if (n_huge == 0) {
@@ -2366,7 +2411,7 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
     if (k >= total) break;
     fold_item(k);
   }
-  const uint32_t row[1] = {(uint32_t)uniq};
+  const uint32_t row[2] = {(uint32_t)tally.uniq, tally.dropped};
   write_stat_row(D, SR_FOLD_B, row);
 }
 

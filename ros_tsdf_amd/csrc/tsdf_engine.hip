@@ -184,6 +184,7 @@ struct tsdf_engine {
   int64_t seg_cap = 0;
   uint32_t* touched = nullptr;
   uint32_t* tile_sums = nullptr;
+  unsigned long long* tile_state = nullptr;
   uint32_t* fail_list = nullptr;
   uint32_t* new_list = nullptr;
   uint8_t* new_map = nullptr;
@@ -254,6 +255,7 @@ struct tsdf_engine {
     D.pcap = pcap;
     D.touched = touched;
     D.tile_sums = tile_sums;
+    D.tile_state = tile_state;
     D.fail_list = fail_list;
     D.new_list = new_list;
     D.new_map = new_map;
@@ -475,14 +477,13 @@ int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int ti
   if (bs.rays_per_lane > 1) hipLaunchKernelGGL(k_bin<true>, dim3(bg), dim3(tb), 0, st, D, inl, bs);
   else hipLaunchKernelGGL(k_bin<false>, dim3(bg), dim3(tb), 0, st, D, inl, bs);
   CHK(timing_mark(E, tid, 1));
-  // (heap assignment runs in k_pass_scan, inside the scan stage: the assign stage is empty)
+  // (heap assignment runs in k_seg, inside the scan stage: the assign stage is empty)
   CHK(timing_mark(E, tid, 2));
   const int64_t tmax = std::min<int64_t>(E->entries, P * stride);
   // tiles of touched blocks are taken grid-stride by at most 256 workgroups
   // (the touched count is only known on the device; most passes have tens)
   const int sg = grid_for(tmax, kScanTile, 256);
-  hipLaunchKernelGGL(k_pass_scan, dim3(sg), dim3(256), 0, st, D);
-  hipLaunchKernelGGL(k_seg_apply, dim3(sg), dim3(256), 0, st, D);
+  hipLaunchKernelGGL(k_seg, dim3(sg), dim3(256), 0, st, D);
   CHK(timing_mark(E, tid, 3));
   if (P > kEmitStagePoints) hipLaunchKernelGGL(k_emit<true>, dim3(grid_for(P, 256, 65536)), dim3(256), 0, st, D);
   else hipLaunchKernelGGL(k_emit<false>, dim3(grid_for(P, 256, 65536)), dim3(256), 0, st, D);
@@ -499,7 +500,6 @@ int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int ti
   static_assert(kFoldBGrid <= kStatRows[SR_FOLD_B], "a statistics row per workgroup");
   E->last_grids.n[SR_BIN] = bg;
   E->last_grids.n[SR_SCAN] = sg;
-  E->last_grids.n[SR_SEG] = sg;
   E->last_grids.n[SR_FOLD_A] = fg;
   E->last_grids.n[SR_FOLD_B] = kFoldBGrid;
   CHK(timing_mark(E, tid, 5));
@@ -844,7 +844,8 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
       (r = dev_alloc(&E->heap_free, c.heap_blocks)) || (r = dev_alloc(&E->freed, c.heap_blocks)) ||
       (r = dev_alloc(&E->desc_dev, 1)) ||
       (r = dev_alloc(&E->touched, entries)) ||
-      (r = dev_alloc(&E->tile_sums, entries / kScanTile + 2)) || (r = dev_alloc(&E->fail_list, entries)) ||
+      (r = dev_alloc(&E->tile_sums, entries / kScanTile + 2)) || (r = dev_alloc(&E->tile_state, entries / kScanTile + 2)) ||
+      (r = dev_alloc(&E->fail_list, entries)) ||
       (r = dev_alloc(&E->new_list, entries)) || (r = dev_alloc(&E->new_map, entries)) ||
       (r = dev_alloc(&E->touched_map, entries)) || (r = dev_alloc(&E->part, ST_COUNT * kPartSlots)) || (r = dev_alloc(&E->srows, (int64_t)kStatRowsTotal * kStatCols)) ||
       (r = dev_alloc(&E->sel, entries)) || (r = dev_alloc(&E->pub_cnt, entries + 1)) ||
@@ -874,6 +875,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
       hipMemsetAsync(E->mctr, 0, sizeof(MapCtr) * kMaxMaps, st) != hipSuccess ||
       hipMemsetAsync(E->heap, 0, (size_t)vbytes, st) != hipSuccess ||
       hipMemsetAsync(E->blk_cnt, 0, sizeof(unsigned long long) * entries, st) != hipSuccess ||
+      hipMemsetAsync(E->tile_state, 0, sizeof(unsigned long long) * (entries / kScanTile + 2), st) != hipSuccess ||
       hipMemsetAsync(E->part, 0, sizeof(unsigned long long) * ST_COUNT * kPartSlots, st) != hipSuccess ||
       hipMemsetAsync(E->srows, 0, sizeof(uint32_t) * kStatRowsTotal * kStatCols, st) != hipSuccess ||
       hipMemsetAsync(E->keys, 0xFF, sizeof(unsigned long long) * entries, st) != hipSuccess ||
@@ -904,7 +906,7 @@ int tsdf_engine_destroy(tsdf_engine* E) {
     }
   void* ptrs[] = {E->maps_dev, E->mctr, E->ctr, E->keys, E->vals, E->blk_cnt, E->blk_base, E->next, E->ovf_stack,
                   E->heap, E->heap_free, E->freed, E->desc_dev, E->pair_e, E->pair_r, E->pair_n,
-                  E->seg, E->touched, E->tile_sums, E->fail_list, E->new_list, E->new_map, E->fold_desc, E->part, E->srows,
+                  E->seg, E->touched, E->tile_sums, E->tile_state, E->fail_list, E->new_list, E->new_map, E->fold_desc, E->part, E->srows,
                   E->touched_map, E->sel,
                   E->pub_cnt, E->pub_off, E->pub_buf, E->in_buf};
   for (void* p : ptrs)
