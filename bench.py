@@ -13,6 +13,8 @@ Also reported on the same JSON line:
                through the plain per-scan path
   fp16         the dense workload with 4-byte voxel state, and its error
                against the FP64 oracle
+  stationary   a parked sensor: one scan integrated again and again, so that
+               many voxels sit at max_weight (the update-by-update fold)
   serving      Poisson arrivals over 16 maps at 400 scans/s: continuous
                batching vs plain per-scan serving, p50/p99 latency
   roofline     the dominant kernel against MI355X HBM bandwidth; its VALU
@@ -117,6 +119,34 @@ def full_size_batches(eng, bmaps_pts, bmaps_sens, knobs):
             "scans_per_launch": per, "points_per_scan": fpts / (2 * per), "ms_per_launch": 1e3 * tfb / 2}
 
 
+def stationary_sensor(warm=120, timed=100):
+    """A parked sensor: the dense workload's first scan integrated again and again on one map through the plain
+    per-scan path.  Hot voxels reach max_weight within tens of scans, so in the timed scans many voxels' updates
+    cross it: the regime where a pass folds the clamped voxels update by update in point order."""
+    from ros_tsdf_amd import Engine, configs, synth
+    p, s, _ = synth.make_scan(configs.DENSE, 0)
+    with Engine(heap_blocks=1 << 18, total_buckets=1 << 20, max_batch_points=1 << 17) as eng:
+        m = eng.map(configs.DENSE_PARAMS, buckets_log2=17)
+        buf = eng.buffer(p.nbytes)
+        buf.upload(p)
+        n = p.shape[0]
+        for _ in range(warm):
+            m.integrate(buf, n, s)
+        eng.synchronize()
+        tot = 0.0
+        for _ in range(timed):
+            t0 = time.perf_counter()
+            m.integrate(buf, n, s)
+            eng.synchronize()
+            tot += time.perf_counter() - t0
+        _, _, w = m.export()
+        sat = float(np.mean(w[w != 0] >= configs.DENSE_PARAMS[2]))
+        m.close()
+        buf.free()
+    return {"points_per_s": n * timed / tot, "ms_per_scan": 1e3 * tot / timed, "scans_before_timing": warm,
+            "scans_timed": timed, "touched_voxels_at_max_weight": sat}
+
+
 def atomics_line(stage_ms):
     """k_bin's global atomics against the device's atomic rate.  The count per launch is a stored profile
     (rocprofv3 TCC_ATOMIC_sum over the plain workload, profiles/atomics_dense.json); its duration is this
@@ -156,7 +186,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed scans of the dense workload")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--only", default="all", choices=["all", "plain", "batched_full"],
+    ap.add_argument("--only", default="all", choices=["all", "plain", "batched_full", "stationary"],
                     help="plain: the headline only; batched_full: only the full-size batched comparison (profiling)")
     ap.add_argument("--error-scans", type=int, default=200,
                     help="dense scans integrated before the FP16/FP32 error figures (also the CPU-baseline sample); "
@@ -181,6 +211,14 @@ def main():
         log("batched full-size", r)
         print(json.dumps({"metric": "batched full-size points/s", "value": r["points_per_s"], "unit": "points/s",
                           "n_gpus": 1, "higher_is_better": True, "knobs": knobs, "batched_full_size": r}))
+        return
+
+    if args.only == "stationary":
+        from ros_tsdf_amd import Engine
+        r = stationary_sensor()
+        log("stationary", r)
+        print(json.dumps({"metric": "stationary-sensor plain per-scan points/s", "value": r["points_per_s"],
+                          "unit": "points/s", "n_gpus": 1, "higher_is_better": True, "stationary": r}))
         return
 
     K, W = args.steps, args.warmup
@@ -321,6 +359,10 @@ def main():
     eng.close()
 
     if extras:
+        # ---------------- stationary sensor: voxels at max_weight ----------------
+        result["stationary"] = stationary_sensor()
+        log("stationary", result["stationary"])
+
         # ---------------- batched: HIP graph, 32 scans from 8 maps ----------------
         plan = workloads.batch_plan(n_batches=8, scans_per_map=16)
         batches = []

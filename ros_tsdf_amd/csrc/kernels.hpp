@@ -44,7 +44,8 @@ constexpr int kScanTile = 1024;  // elements per workgroup in the scan kernels
 // Size classes of touched blocks (records per block): the fold kernel and
 // the number of blocks folded together differ per class.
 // FC_DEFER holds blocks of FC_WG that k_fold_a hands to k_fold_b (a group of
-// theirs would cross max_weight); no block is filed there by size.
+// theirs would cross max_weight and they hold more records than k_fold_a's
+// ordered tile, kOrdTileA); no block is filed there by size alone.
 enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WGL, FC_WG, FC_WG2, FC_HUGE, FC_DEFER, FC_COUNT };
 constexpr int kWgSplit = 128;  // FC_WGL: 65..128 records (a wave each), FC_WG: 129..kCountTileS
 static_assert(FC_COUNT <= (int)(sizeof(EngCtr::fold_n) / sizeof(int32_t)), "fold class counters");
@@ -2268,22 +2269,44 @@ __device__ void fold_point_rounds(const Dev& D, const int4 d, unsigned long long
   __syncthreads();
 }
 
+// k_fold_a's dynamic LDS: gw windows of 32-bit voxel sums (4 KB each) and
+// the map's group starts after them for the sum path; for the ordered fold of
+// a block of at most kOrdTileA records, its counting tile pair, counts,
+// voxel starts and scan scratch over the same bytes (the sum path is over by
+// then).  The wave tier's strips use the first 4 KB.
+constexpr int kOrdTileA = 768;
+constexpr int kOrdCntA = 2 * kOrdTileA * 8;                        // byte offset of the counts
+constexpr int kOrdStartA = kOrdCntA + kVoxelsPerBlock * 4;         // of the voxel starts (16-bit)
+constexpr int kOrdShA = kOrdStartA + ((kVoxelsPerBlock + 1) * 2 + 3) / 4 * 4;  // of the scan scratch
+constexpr int kOrdBytesA = kOrdShA + kScanSh * 4;
+__host__ __device__ constexpr int fold_a_lds_bytes(int gw) {
+  return gw * kVoxelsPerBlock * 8 + kMaxScansPerBatch * 4 > kOrdBytesA ? gw * kVoxelsPerBlock * 8 + kMaxScansPerBatch * 4
+                                                                         : kOrdBytesA;
+}
+static_assert(kOrdTileA % 256 == 0 && kOrdTileA <= kCountTileS, "a whole number of records per thread");
+
 // Fold, first launch.  Work items, in this order: the blocks of 129..1024
 // records (a workgroup each, by sums; a block of which some group would
-// cross max_weight is filed under FC_DEFER for k_fold_b), those of 65..128
-// (a wave each), then chunks of the small classes (one group of 64/G blocks
-// per wave).  LDS: gw windows of 32-bit voxel sums (4 KB each, the launch's
-// dynamic LDS; the wave tier's strips share it) and the pass's scans.  At most 64
-// VGPRs per thread let eight workgroups share a CU.  Workgroup q takes items
-// q, q + grid, ...; with a grid larger than what fits on the chip at once,
-// the dispatcher starts a new workgroup whenever one finishes, which
-// balances the uneven items.
+// cross max_weight is folded in order in place when it holds at most
+// kOrdTileA records, else filed under FC_DEFER for k_fold_b), those of
+// 65..128 (a wave each), then chunks of the small classes (one group of 64/G
+// blocks per wave).  LDS: fold_a_lds_bytes(gw) of dynamic LDS and the pass's
+// scans; at most 64 VGPRs per thread, so eight workgroups share a CU.
+// Workgroup q takes items q, q + grid, ...; with a grid larger than what fits
+// on the chip at once, the dispatcher starts a new workgroup whenever one
+// finishes, which balances the uneven items.
 template <class State>
 __global__ void __launch_bounds__(256, 8) k_fold_a(Dev D, int gw) {
-  extern __shared__ uint2 fold_a_acc[];  // gw * 512 32-bit sums
+  extern __shared__ __align__(16) unsigned char fold_a_lds[];
   __shared__ PassScans ps;
-  __shared__ int32_t bnd[kMaxScansPerBatch];
   __shared__ int flag;
+  uint2* const acc = reinterpret_cast<uint2*>(fold_a_lds);
+  int32_t* const bnd = reinterpret_cast<int32_t*>(fold_a_lds + gw * kVoxelsPerBlock * sizeof(uint2));
+  unsigned long long* const rin = reinterpret_cast<unsigned long long*>(fold_a_lds);
+  unsigned long long* const rout = rin + kOrdTileA;
+  uint32_t* const cnt = reinterpret_cast<uint32_t*>(fold_a_lds + kOrdCntA);
+  uint16_t* const ostart = reinterpret_cast<uint16_t*>(fold_a_lds + kOrdStartA);
+  uint32_t* const osh = reinterpret_cast<uint32_t*>(fold_a_lds + kOrdShA);
   FoldTally tally;
   stage_scans(D, ps);
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG]);
@@ -2300,21 +2323,25 @@ __global__ void __launch_bounds__(256, 8) k_fold_a(Dev D, int gw) {
   const int total = n_wg + cl + c64 + c32 + c16;
   const int4* dwg = D.fold_desc + D.fold_base[FC_WG];
   const int4* dwgl = D.fold_desc + D.fold_base[FC_WGL];
-  static_assert(kVoxelsPerBlock * sizeof(uint2) >= 4 * kWgSplit * sizeof(unsigned long long),
-                "the wave strips fit one window of sums");
+  static_assert(4 * kWgSplit * sizeof(unsigned long long) <= kVoxelsPerBlock * sizeof(uint2),
+                "the wave strips fit the first window of sums");
   static_assert(kCountTileS <= kFixSum32, "the workgroup tier's sums fit 32 bits");
   __syncthreads();
   for (int it = blockIdx.x; it < total; it += gridDim.x) {
     if (it < n_wg) {
       const int4 d0 = dwg[it];
-      if (fold_sum_block<State, 256, false>(D, resolve_desc(D, d0), fold_a_acc, gw, ps, bnd, &flag, tally) &&
-          threadIdx.x == 0)
-        D.fold_desc[D.fold_base[FC_DEFER] + atomicAdd(&D.ctr->fold_n[FC_DEFER], 1)] = d0;
+      const int4 d = resolve_desc(D, d0);
+      if (fold_sum_block<State, 256, false>(D, d, acc, gw, ps, bnd, &flag, tally)) {
+        if (d.y <= kOrdTileA)
+          fold_count_block<State, kOrdTileA, 256, uint16_t>(D, d, rin, rout, cnt, ostart, osh, ps, tally);
+        else if (threadIdx.x == 0)
+          D.fold_desc[D.fold_base[FC_DEFER] + atomicAdd(&D.ctr->fold_n[FC_DEFER], 1)] = d0;
+      }
     } else if (it < n_wg + cl) {
       const int q = (it - n_wg) * kl + wave;
       if (q < n_wgl) {
         const int4 d = resolve_desc(D, dwgl[q]);
-        unsigned long long* strip = reinterpret_cast<unsigned long long*>(fold_a_acc) + wave * kWgSplit;
+        unsigned long long* strip = reinterpret_cast<unsigned long long*>(fold_a_lds) + wave * kWgSplit;
         fold_wave_block<State, kWgSplit / 64>(D, d, strip, ps, tally);
       }
       __syncthreads();  // the strips live in the sums' LDS, which the next item may use

@@ -493,7 +493,7 @@ int enqueue_pass(tsdf_engine* E, const Dev& D, int64_t P, int64_t stride, int ti
   with_state(E->state_format, [&](auto s) {
     using State = decltype(s);
     gw = std::min(std::max(gw, 1), kFoldAWindows);
-    hipLaunchKernelGGL(k_fold_a<State>, dim3(fg), dim3(256), (size_t)gw * kVoxelsPerBlock * sizeof(uint2), st, D, gw);
+    hipLaunchKernelGGL(k_fold_a<State>, dim3(fg), dim3(256), (size_t)fold_a_lds_bytes(gw), st, D, gw);
     hipLaunchKernelGGL(k_fold_b<State>, dim3(kFoldBGrid), dim3(kFoldBThreads), 0, st, D);
     return 0;
   });
