@@ -1635,6 +1635,14 @@ __device__ bool fold_sum_block(const Dev& D, const int4 d, void* acc, int gw, co
     if (t == 0) tally.dropped += (uint32_t)c;
     return false;
   }
+  // every voxel state of the block is requested now, with the records: a
+  // block of this tier touches a voxel in nearly every 64-byte line of its
+  // states, and the loads then overlap the sums instead of following them
+#pragma unroll
+  for (int u = 0; u < kVpt; ++u) {
+    const int l = t + NT * u;
+    if (l < kVoxelsPerBlock) State::load(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, S[u], W[u]);
+  }
   for (int wb = 0;; wb += gw) {
     const int nw = min(gw, G - wb);
     for (int x = t; x < nw * kVoxelsPerBlock; x += NT) accp[x] = Pair{0, 0};
@@ -1666,17 +1674,8 @@ __device__ bool fold_sum_block(const Dev& D, const int4 d, void* acc, int gw, co
       }
     }
     __syncthreads();
-    // each voxel's groups of this turn, in order: the states of the voxels
-    // first touched now are requested together, then the groups applied
+    // each voxel's groups of this turn, in order
     bool clamp = false;
-#pragma unroll
-    for (int u = 0; u < kVpt; ++u) {
-      const int l = t + NT * u;
-      if (l >= kVoxelsPerBlock) continue;
-      bool any = false;
-      for (int g = 0; g < nw; ++g) any = any || accp[g * kVoxelsPerBlock + l].x != 0;
-      if (any && !seen[u]) State::load(D.heap, (int64_t)d.z * kVoxelsPerBlock + l, cd, S[u], W[u]);
-    }
 #pragma unroll
     for (int u = 0; u < kVpt; ++u) {
       const int l = t + NT * u;
