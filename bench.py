@@ -52,9 +52,9 @@ def make_pool(scans):
     return Pool(scans)
 
 
-def plain_throughput(eng, m, buf, pool, idx, with_stats=True):
+def plain_throughput(eng, m, buf, pool, idx, with_stats=True, times=None):
     """Integrates pool scans idx one at a time, synchronising after each.
-    Returns (seconds summed over steps, points, per-step stats)."""
+    Returns (seconds summed over steps, points, per-step stats); times, if given, receives each step's seconds."""
     total = 0.0
     pts = 0
     stats = []
@@ -63,7 +63,10 @@ def plain_throughput(eng, m, buf, pool, idx, with_stats=True):
         t0 = time.perf_counter()
         m.integrate(buf, pool.counts[i], pool.sensors[i], offset_points=pool.offsets[i])
         eng.synchronize()
-        total += time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        total += dt
+        if times is not None:
+            times.append(dt)
         pts += pool.counts[i]
         if with_stats:
             stats.append(eng.stats())
@@ -246,7 +249,11 @@ def main():
     buf.upload(dpool.points)
     order = [k % need for k in range(W + K)]
     plain_throughput(eng, m, buf, dpool, order[:W], with_stats=False)
-    secs, npts, _ = plain_throughput(eng, m, buf, dpool, order[W:], with_stats=False)
+    step_s = []
+    secs, npts, _ = plain_throughput(eng, m, buf, dpool, order[W:], with_stats=False, times=step_s)
+    q = np.array(step_s) * 1e3
+    log("plain step ms: median %.4f, first 20 %.4f, last 20 %.4f, p99 %.4f"
+        % (float(np.median(q)), float(np.mean(q[:20])), float(np.mean(q[-20:])), float(np.percentile(q, 99))))
     # stage timings and per-scan statistics: the same scans again on a fresh map,
     # with the HIP timing events on (kept out of the timed loop above)
     mt = eng.map(configs.DENSE_PARAMS, buckets_log2=17)
