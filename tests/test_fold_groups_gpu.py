@@ -3,10 +3,10 @@
 A group's updates are summed exactly (fixed point) and applied as one weighted mean while the voxel's
 weight stays within max_weight; from the update that would cross it, the group is folded update by update
 in point order.  The cases below put many updates on few voxels of one block in one pass, in every fold tier
-(16/17 .. 12606 records per block, the bounds of test_edges_gpu's tier cases), with a max_weight that groups
-cross inside the pass (4) or never reach (1000), and with 1, 3 or 10 scans of the map in the pass.  Each map
-is compared with the FP64 oracle, which folds update by update in point order, and the batched passes with
-the same scans one pass each, bit for bit, in all three state formats.
+(16/17 .. 12606 records per block, the bounds of test_edges_gpu's tier cases, and 768/769), with a max_weight
+that groups cross inside the pass (4) or never reach (1000), and with 1, 3 or 10 scans of the map in the pass.
+Each map is compared with the FP64 oracle, which folds update by update in point order, and the batched passes
+with the same scans one pass each, bit for bit, in all three state formats.
 """
 import numpy as np
 import pytest
@@ -62,11 +62,13 @@ def run(eng, params, scans, passes, batched):
     return m, st
 
 
+# (768, 1) and (769, 1): the largest block whose clamped groups the first fold launch folds in order itself,
+# and the smallest it hands to the second launch
 CASES = [(n, cols, ns, maxw)
-         for (n, cols) in [(16, 1), (17, 1), (64, 1), (65, 1), (128, 1), (129, 1), (1024, 1), (1025, 1), (4096, 1),
-                           (4097, 2), (8193, 2), (12606, 1)]
+         for (n, cols) in [(16, 1), (17, 1), (64, 1), (65, 1), (128, 1), (129, 1), (768, 1), (769, 1), (1024, 1),
+                           (1025, 1), (4096, 1), (4097, 2), (8193, 2), (12606, 1)]
          for (ns, maxw) in [(1, 4.0), (3, 4.0), (3, 1000.0)]]
-CASES += [(1024, 1, 10, 4.0), (1024, 1, 10, 1000.0), (4096, 1, 10, 4.0), (4096, 1, 10, 1000.0),
+CASES += [(768, 1, 10, 4.0), (1024, 1, 10, 4.0), (1024, 1, 10, 1000.0), (4096, 1, 10, 4.0), (4096, 1, 10, 1000.0),
           (8193, 2, 10, 1000.0)]
 
 
