@@ -43,10 +43,13 @@ constexpr int kScanTile = 1024;  // elements per workgroup in the scan kernels
 
 // Size classes of touched blocks (records per block): the fold kernel and
 // the number of blocks folded together differ per class.
-// FC_DEFER holds blocks of FC_WG that k_fold_a hands to k_fold_b (a group of
-// theirs would cross max_weight and they hold more records than k_fold_a's
-// ordered tile, kOrdTileA); no block is filed there by size alone.
-enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WGL, FC_WG, FC_WG2, FC_HUGE, FC_DEFER, FC_COUNT };
+enum FoldClass { FC_16 = 0, FC_32, FC_64, FC_WGL, FC_WG, FC_WG2, FC_HUGE, FC_COUNT };
+// Set by k_fold_a in the map field of a descriptor it hands to k_fold_b (a
+// group of the block would cross max_weight and the block holds more records
+// than k_fold_a's ordered tile, kOrdTileA): a plain store, where a shared
+// list would put hundreds of returning atomics on one counter.  k_seg writes
+// every descriptor of a pass afresh, without it.
+constexpr int32_t kDescDeferred = 1 << 30;
 constexpr int kWgSplit = 128;  // FC_WGL: 65..128 records (a wave each), FC_WG: 129..kCountTileS
 static_assert(FC_COUNT <= (int)(sizeof(EngCtr::fold_n) / sizeof(int32_t)), "fold class counters");
 
@@ -54,7 +57,7 @@ static_assert(FC_COUNT <= (int)(sizeof(EngCtr::fold_n) / sizeof(int32_t)), "fold
 // blocks and `records` records: every block of class c holds at least lo
 // records (the class's lower size bound).
 __host__ __device__ constexpr int64_t fold_class_cap(int c, int64_t blocks, int64_t records) {
-  const int64_t lo = c == FC_16 ? 1 : c == FC_32 ? 17 : c == FC_64 ? 33 : c == FC_WGL ? 65 : (c == FC_WG || c == FC_DEFER) ? kWgSplit + 1 : c == FC_WG2 ? kCountTileS + 1 : kCountTile + 1;
+  const int64_t lo = c == FC_16 ? 1 : c == FC_32 ? 17 : c == FC_64 ? 33 : c == FC_WGL ? 65 : c == FC_WG ? kWgSplit + 1 : c == FC_WG2 ? kCountTileS + 1 : kCountTile + 1;
   const int64_t r = records / lo + 1;
   return r < blocks ? r : blocks;
 }
@@ -2284,12 +2287,14 @@ __host__ __device__ constexpr int fold_a_lds_bytes(int gw) {
 }
 static_assert(kOrdTileA % 256 == 0 && kOrdTileA <= kCountTileS, "a whole number of records per thread");
 
-// Fold, first launch.  Work items, in this order: the blocks of 129..1024
-// records (a workgroup each, by sums; a block of which some group would
-// cross max_weight is folded in order in place when it holds at most
-// kOrdTileA records, else filed under FC_DEFER for k_fold_b), those of
-// 65..128 (a wave each), then chunks of the small classes (one group of 64/G
-// blocks per wave).  LDS: fold_a_lds_bytes(gw) of dynamic LDS and the pass's
+// Fold, first launch.  Work items, in this order: in a pass of one scan, the
+// blocks of kCountTileS+1..kCountTile records (a workgroup each, by 64-bit
+// sums in one window; a block of which some group would cross max_weight is
+// marked kDescDeferred for k_fold_b); the blocks of 129..1024 records (a
+// workgroup each, by 32-bit sums; a block of which some group would cross
+// max_weight is folded in order in place when it holds at most kOrdTileA
+// records, else marked kDescDeferred), those of 65..128 (a wave each), then
+// chunks of the small classes (one group of 64/G blocks per wave).  LDS: fold_a_lds_bytes(gw) of dynamic LDS and the pass's
 // scans; at most 64 VGPRs per thread, so eight workgroups share a CU.
 // Workgroup q takes items q, q + grid, ...; with a grid larger than what fits
 // on the chip at once, the dispatcher starts a new workgroup whenever one
@@ -2308,6 +2313,7 @@ __global__ void __launch_bounds__(256, 8) k_fold_a(Dev D, int gw) {
   uint32_t* const osh = reinterpret_cast<uint32_t*>(fold_a_lds + kOrdShA);
   FoldTally tally;
   stage_scans(D, ps);
+  const int n_wg2 = ld_acq(&D.ctr->fold_n[FC_WG2]);
   const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG]);
   const int n_wgl = ld_acq(&D.ctr->fold_n[FC_WGL]);
   const int n64 = ld_acq(&D.ctr->fold_n[FC_64]);
@@ -2319,14 +2325,28 @@ __global__ void __launch_bounds__(256, 8) k_fold_a(Dev D, int gw) {
   const int kl = waves, k64 = waves * 1, k32 = waves * 2, k16 = waves * 4;
   const int cl = (n_wgl + kl - 1) / kl;
   const int c64 = (n64 + k64 - 1) / k64, c32 = (n32 + k32 - 1) / k32, c16 = (n16 + k16 - 1) / k16;
-  const int total = n_wg + cl + c64 + c32 + c16;
+  const int4* dwg2 = D.fold_desc + D.fold_base[FC_WG2];
   const int4* dwg = D.fold_desc + D.fold_base[FC_WG];
   const int4* dwgl = D.fold_desc + D.fold_base[FC_WGL];
   static_assert(4 * kWgSplit * sizeof(unsigned long long) <= kVoxelsPerBlock * sizeof(uint2),
                 "the wave strips fit the first window of sums");
   static_assert(kCountTileS <= kFixSum32, "the workgroup tier's sums fit 32 bits");
+  static_assert(kVoxelsPerBlock * sizeof(ulonglong2) <= kOrdBytesA, "one window of 64-bit sums fits");
   __syncthreads();
-  for (int it = blockIdx.x; it < total; it += gridDim.x) {
+  // a pass of one scan (one group per voxel: one window of sums) folds its
+  // blocks of kCountTileS+1..kCountTile records here as well, first, as its
+  // longest items; k_fold_b, which takes them otherwise, makes the same test
+  const int nb = ps.ns == 1 ? n_wg2 : 0;
+  const int total = nb + n_wg + cl + c64 + c32 + c16;
+  for (int it0 = blockIdx.x; it0 < total; it0 += gridDim.x) {
+    if (it0 < nb) {
+      const int4 d0 = dwg2[it0];
+      if (fold_sum_block<State, 256, true>(D, resolve_desc(D, d0), acc, 1, ps, bnd, &flag, tally) &&
+          threadIdx.x == 0)
+        D.fold_desc[D.fold_base[FC_WG2] + it0].w = d0.w | kDescDeferred;
+      continue;
+    }
+    const int it = it0 - nb;
     if (it < n_wg) {
       const int4 d0 = dwg[it];
       const int4 d = resolve_desc(D, d0);
@@ -2334,7 +2354,7 @@ __global__ void __launch_bounds__(256, 8) k_fold_a(Dev D, int gw) {
         if (d.y <= kOrdTileA)
           fold_count_block<State, kOrdTileA, 256, uint16_t>(D, d, rin, rout, cnt, ostart, osh, ps, tally);
         else if (threadIdx.x == 0)
-          D.fold_desc[D.fold_base[FC_DEFER] + atomicAdd(&D.ctr->fold_n[FC_DEFER], 1)] = d0;
+          D.fold_desc[D.fold_base[FC_WG] + it].w = d0.w | kDescDeferred;
       }
     } else if (it < n_wg + cl) {
       const int q = (it - n_wg) * kl + wave;
@@ -2364,8 +2384,10 @@ __global__ void __launch_bounds__(256, 8) k_fold_a(Dev D, int gw) {
 }
 
 // Fold, second launch (which also takes the pass's failed inserts out of the
-// table): blocks of kCountTileS+1..kCountTile records and the larger ones, by sums (a 1024-thread workgroup each, the 64 KB buffer as
-// eight windows of voxel sums), and the blocks k_fold_a deferred.  A block of
+// table): blocks of kCountTileS+1..kCountTile records (in a pass of more than
+// one scan; k_fold_a takes them in a pass of one) and the larger ones, by
+// sums (a 1024-thread workgroup each, the 64 KB buffer as eight windows of
+// voxel sums), and the blocks k_fold_a marked kDescDeferred.  A block of
 // which some group would cross max_weight takes the ordered fold, with the
 // buffer as the counting tile pair: a counting sort by voxel and rank
 // placement by point, in voxel ranges that fit the tile for the larger
@@ -2373,7 +2395,7 @@ __global__ void __launch_bounds__(256, 8) k_fold_a(Dev D, int gw) {
 // Eight waves per SIMD (two workgroups, and their 2 x 68 KB of LDS, per CU)
 // cap the kernel at 64 VGPRs.
 // Items are handed out one at a time from a pass counter, largest first
-// (the ranged blocks, then the kCountTile class, then the deferred ones):
+// (the ranged blocks, then the kCountTile class):
 // blocks differ in cost by an order of magnitude, and a fixed stride over
 // them left some workgroups with several of the largest while the rest had
 // finished.  A pass without ranged blocks (a single scan's: fewer items than
@@ -2390,31 +2412,30 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
   __shared__ uint32_t rstart[kMaxRanges + 1];
   __shared__ PassScans ps;
   __shared__ int32_t bnd[kMaxScansPerBatch];
-  __shared__ int next_item, flag;
+  __shared__ int next_item, flag, n_marked;
+  __shared__ int marked[kFoldBThreads];
   constexpr int kSumWindows = (int)(kFoldCap * sizeof(unsigned long long) / (kVoxelsPerBlock * sizeof(ulonglong2)));
   FoldTally tally;
   stage_scans(D, ps);
-  const int n_wg = ld_acq(&D.ctr->fold_n[FC_WG2]);
+  const int n_wg2 = ld_acq(&D.ctr->fold_n[FC_WG2]);
   const int n_huge = ld_acq(&D.ctr->fold_n[FC_HUGE]);
-  const int n_def = ld_acq(&D.ctr->fold_n[FC_DEFER]);
+  const int n_wg1 = ld_acq(&D.ctr->fold_n[FC_WG]);
   const int4* desc = D.fold_desc + D.fold_base[FC_WG2];
   const int4* hdesc = D.fold_desc + D.fold_base[FC_HUGE];
-  const int4* ddesc = D.fold_desc + D.fold_base[FC_DEFER];
+  const int4* wdesc = D.fold_desc + D.fold_base[FC_WG];
   __syncthreads();
+  // (k_fold_a folds the FC_WG2 blocks of a one-scan pass)
+  const int n_wg = ps.ns == 1 ? 0 : n_wg2;
   auto fold_item = [&](int k) {
     if (k < n_huge) {
       const int4 d = resolve_desc(D, hdesc[k]);
       if (fold_sum_block<State, kFoldBThreads, true>(D, d, buf, kSumWindows, ps, bnd, &flag, tally) &&
           !fold_ranged_block<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, rstart, sh, ps, tally))
         fold_point_rounds<State, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, sh, ps, tally);
-    } else if (k < n_huge + n_wg) {
+    } else {
       const int4 d = resolve_desc(D, desc[k - n_huge]);
       if (fold_sum_block<State, kFoldBThreads, true>(D, d, buf, kSumWindows, ps, bnd, &flag, tally))
         fold_count_block<State, kCountTile, kFoldBThreads>(D, d, buf, buf + kCountTile, cnt, start, sh, ps, tally);
-    } else {
-      // (k_fold_a deferred it after it found its heap block)
-      fold_count_block<State, kCountTile, kFoldBThreads>(D, resolve_desc(D, ddesc[k - n_huge - n_wg]), buf,
-                                                          buf + kCountTile, cnt, start, sh, ps, tally);
     }
   };
   // entries inserted in this pass that got no heap block leave the table
@@ -2422,7 +2443,7 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
   const int nf = ld_acq(&D.ctr->fail_n);
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nf; q += gridDim.x * blockDim.x)
     D.keys[D.fail_list[q]] = kEmptyKey;
-  const int total = n_huge + n_wg + n_def;
+  const int total = n_huge + n_wg;
   int q0 = 0;  // the first item taken from the counter
   if (n_huge == 0) {
     if ((int)blockIdx.x < total) fold_item((int)blockIdx.x);
@@ -2436,6 +2457,27 @@ __global__ void __launch_bounds__(kFoldBThreads, 8) k_fold_b(Dev D) {
     __syncthreads();
     if (k >= total) break;
     fold_item(k);
+  }
+  // The blocks k_fold_a marked, folded in order: the FC_WG list and, in a
+  // pass of one scan, the FC_WG2 list, as one index range, dealt out by a
+  // stride of the grid (workgroup w checks w, w + grid, ... with one load per
+  // thread), so the few marked blocks of a pass spread over the workgroups.
+  const int nm = n_wg1 + (ps.ns == 1 ? n_wg2 : 0);
+  for (int r0 = 0; r0 < nm; r0 += (int)gridDim.x * kFoldBThreads) {
+    if (threadIdx.x == 0) n_marked = 0;
+    __syncthreads();
+    const int x = r0 + (int)blockIdx.x + (int)threadIdx.x * (int)gridDim.x;
+    if (x < nm && ((x < n_wg1 ? wdesc[x].w : desc[x - n_wg1].w) & kDescDeferred)) marked[atomicAdd(&n_marked, 1)] = x;
+    __syncthreads();
+    const int nk = n_marked;
+    for (int j = 0; j < nk; ++j) {
+      const int y = marked[j];
+      int4 d = y < n_wg1 ? wdesc[y] : desc[y - n_wg1];
+      d.w &= ~kDescDeferred;
+      fold_count_block<State, kCountTile, kFoldBThreads>(D, resolve_desc(D, d), buf, buf + kCountTile, cnt, start, sh,
+                                                          ps, tally);
+    }
+    __syncthreads();
   }
   const uint32_t row[2] = {(uint32_t)tally.uniq, tally.dropped};
   write_stat_row(D, SR_FOLD_B, row);
