@@ -122,6 +122,7 @@ struct Dev {
   uint32_t* new_list;        // [entries] entries inserted in this pass (heap blocks assigned after binning)
   uint8_t* new_map;          // [entries] their map slots
   int4* fold_desc;           // per size class c, from fold_base[c]: {record base, records, entry, map}
+                             // (map | kDescDeferred: handed from k_fold_a to k_fold_b)
   unsigned long long* part;  // [ST_COUNT][kPartSlots] statistics partial sums (publish and gc)
   uint32_t* srows;           // [kStatRowsTotal][kStatCols] pass statistics, a row per workgroup
   int64_t fold_cap;
