@@ -76,6 +76,26 @@ def test_dense_workload_parity(eng, oracle_lib):
     print("dense:", r[-1])
 
 
+def test_parked_sensor_saturates_like_oracle(eng, oracle_lib):
+    """A parked sensor: the first dense scan integrated six times at max_weight 8, so that the voxels of blocks
+    of every size class reach max_weight within a few passes and their updates take the update-by-update fold
+    from there: in place in the first fold launch for blocks of up to 768 records, handed to the second
+    launch for larger ones.  Every pass is compared with the FP64 oracle."""
+    import numpy as np
+    params = (0.1, 0.4, 8.0, 2500.0, 22500.0)
+    w = configs.DENSE
+    first = next(iter(synth.scans(w, stop=1)))
+    r = run_sequence(eng, oracle_lib, params, w, [first] * 6, publish=False, gc=False)
+    print("parked:", r[-1])
+    om = oracle_lib.OracleMap(params)
+    for _ in range(6):
+        om.integrate(first[1], first[2])
+    _, _, wts = om.export()
+    om.close()
+    wts = np.asarray(wts)
+    assert np.mean(wts[wts > 0] >= params[2]) > 0.2, "too few voxels reached max_weight"
+
+
 def test_tiny_hash_overflow_chains(eng, oracle_lib):
     """4 home buckets force long overflow chains (and chain-extension races)."""
     w = configs.SMALL
