@@ -101,6 +101,46 @@ def test_batched_dense_pass_equals_per_scan():
             m.close()
 
 
+def test_batched_parked_sensor_saturates_like_oracle(oracle_lib):
+    """A parked sensor in batches: four copies of the first dense scan per pass of one map, two passes, at
+    max_weight 8.  A pass puts four scans' records on each block (blocks of more than 4096 records take the
+    voxel-range and point-round folds), and more than a quarter of the voxels reach max_weight, so those
+    folds, the 1025-4096-record tier and the in-place tier all run their update-by-update paths.  The map
+    matches the FP64 oracle fed the same eight scans, and equals the per-scan path's bit for bit."""
+    from ros_tsdf_amd import Engine
+    params = (0.1, 0.4, 8.0, 2500.0, 22500.0)
+    first = next(iter(synth.scans(configs.DENSE, stop=1)))
+    pts, s = first[1], first[2]
+    pool = workloads.Pool([(pts, s, 0)] * 4)
+    with Engine(heap_blocks=1 << 17, total_buckets=1 << 18, max_batch_points=1 << 20) as e:
+        buf = e.buffer(pool.points.nbytes)
+        buf.upload(pool.points)
+        mb = e.map(params, buckets_log2=16)
+        ms = e.map(params, buckets_log2=16)
+        descs = [(0, pool.counts[i], pool.offsets[i], pool.sensors[i]) for i in range(len(pool))]
+        for _ in range(2):
+            e.integrate_batch([mb], descs, buf)
+            e.synchronize()
+            st = e.stats()
+            assert st["updates_dropped"] == 0 and st["insert_failures"] == 0
+            for (_, n, off, sv) in descs:
+                ms.integrate(buf, n, sv, offset_points=off)
+        e.synchronize()
+        assert exports_equal(mb, ms)
+        om = oracle_lib.OracleMap(params)
+        for _ in range(8):
+            om.integrate(pts, s)
+        out = compare_maps(mb, om, params[1], configs.SDF_TOL_T, configs.WEIGHT_TOL_REL * params[2])
+        _, _, w = mb.export()
+        w = np.asarray(w)
+        assert np.mean(w[w > 0] >= params[2]) > 0.25, "too few voxels reached max_weight"
+        print("batched parked:", out)
+        om.close()
+        mb.close()
+        ms.close()
+        buf.free()
+
+
 def test_graph_replays_match_eager(eng):
     from ros_tsdf_amd import BatchGraph
     scans = small_scans(2, 6)
