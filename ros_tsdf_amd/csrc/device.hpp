@@ -71,8 +71,7 @@ struct EngCtr {
   int32_t new_n;      // entries inserted in this pass, waiting for heap blocks
   int32_t fold_next;  // large-block fold items handed out beyond the first per workgroup
   int32_t bin_next;   // k_bin ray chunks handed out from the work queue beyond the first per workgroup
-  int32_t pass_epoch; // passes so far (tags the look-back states of the pass scan)
-  int32_t pad_e;
+  uint32_t seg_top;   // records given out to the pass's block segments (k_seg, one atomic per tile)
   int32_t fold_n[8];  // touched blocks per fold size class
   unsigned long long stats[ST_COUNT];
 };
@@ -117,7 +116,6 @@ struct Dev {
   uint32_t* touched;         // [entries]
   uint8_t* touched_map;      // [entries] map slot of each touched block
   uint32_t* tile_sums;       // [entries / 1024 + 2]
-  unsigned long long* tile_state;  // [entries / 1024 + 2] look-back states of the pass scan (k_seg)
   uint32_t* fail_list;       // [entries] inserted entries without a heap block (removed after binning)
   uint32_t* new_list;        // [entries] entries inserted in this pass (heap blocks assigned after binning)
   uint8_t* new_map;          // [entries] their map slots

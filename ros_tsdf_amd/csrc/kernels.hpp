@@ -9,7 +9,7 @@
 //   k_seg            heap blocks for the new entries (one pop per wave);
 //                    record segment offset and 16-byte fold descriptor of
 //                    every touched block, filed by size class (a scan over
-//                    the touched list with decoupled look-back)
+//                    the touched list, one segment atomic per tile)
 //   k_emit           per point: second voxel-DDA walk writing each update
 //                    record (voxel, point, sdf) to its block's contiguous
 //                    record segment at the offset reserved in k_bin
@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(kPassStatsThreads) k_pass_stats(Dev D, StatGri
     case ST_TRUNC_BLK: reg[0] = SR_BIN; col[0] = 5; break;
     case ST_INSERT_FAIL: reg[0] = SR_BIN; col[0] = 6; break;
     case ST_UPD_DROPPED: reg[0] = SR_BIN; col[0] = 7; reg[1] = SR_FOLD_A; col[1] = 1; reg[2] = SR_FOLD_B; col[2] = 1;
-      reg[3] = SR_SCAN; col[3] = 3; break;
+      break;
     case ST_NEW_BLOCKS: reg[0] = SR_SCAN; col[0] = 0; break;
     case ST_HEAP_FAIL: reg[0] = SR_SCAN; col[0] = 1; break;
     case ST_QUOTA_FAIL: reg[0] = SR_SCAN; col[0] = 2; break;
@@ -634,13 +634,13 @@ __global__ void __launch_bounds__(256, 3) k_bin(Dev D, PassInline inl, BinSched 
   const PassHdr hdr = inl.nscans >= 0 ? inl.hdr : *D.hdr;
   const int P = hdr.total;
   if (blockIdx.x == 0) {
-    // the previous pass's fold is over: clear its counters, start a new
-    // pass epoch; store the inline descriptors for k_emit
+    // the previous pass's fold is over: clear its counters; store the inline
+    // descriptors for k_emit
     const int t = threadIdx.x;
     if (t == 0) {
       D.ctr->fold_next = 0;
       D.ctr->fail_n = 0;  // (its last reader, k_fold_b, is over)
-      D.ctr->pass_epoch++;
+      D.ctr->seg_top = 0;  // (its last user, k_seg, is over)
     }
     if (t < FC_COUNT) D.ctr->fold_n[t] = 0;
     if (inl.nscans >= 0) {
@@ -1070,47 +1070,30 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int32_t* n_ptr, Count 
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_sums[ntiles];
 }
 
-// Look-back state of one tile of the touched list in the pass scan: [pass
-// epoch (30 bits) | kind (2 bits) | records (32 bits)], one 64-bit word, so
-// a state and its value are published together.  kind 1: the tile's own
-// records; kind 2: the records of the tiles up to it.  A word of another
-// epoch is a previous pass's: not published yet in this one.
-constexpr uint32_t kTileOwn = 1, kTileUpTo = 2;
-constexpr int kTileSpin = 1 << 22;  // bound on waiting for a predecessor (never reached: see k_seg)
-__device__ __forceinline__ unsigned long long tile_word(uint32_t epoch, uint32_t kind, uint32_t v) {
-  return ((unsigned long long)(((epoch & 0x3FFFFFFFu) << 2) | kind) << 32) | v;
-}
-
 // The pass scan, one launch after binning:
 //  * heap blocks for the pass's new entries, 256 per workgroup and chunk
 //    (taken from the last workgroup down, so the tile workgroups get them
 //    last); off the free stack with one pop, and one per-map count, per wave;
 //  * record segments and fold descriptors of the touched blocks, per tile of
-//    the touched list: the tile publishes its record total, then sums the
-//    records of the tiles before it from its predecessors' published states
-//    (decoupled look-back: a predecessor's "up to" total ends the walk, its
-//    own total adds and the walk goes on) and publishes its own "up to"
-//    total.  Each block's segment offset follows by a workgroup scan; its
-//    16-byte descriptor {record base, records, entry, map} is filed under its
-//    size class with one counter atomic per class and workgroup, and its pass
-//    counter is reset.  The descriptor holds the entry, not its heap block:
-//    the heap blocks of new entries are being assigned beside it, and the
-//    fold reads them.
-// A workgroup waits only for tiles of lower index, which lower workgroups
-// take in the same or an earlier turn, and the grid (at most 256 workgroups
-// of 256 threads) is resident at once: every wait ends.  The bound on it is
-// a guard only; a tile that reached it would count its records as dropped.
+//    the touched list: a workgroup scan of the tile's record counts, and one
+//    atomic on the pass's segment counter for the tile's total, which places
+//    the tile's segments after those of the tiles served before it (in any
+//    order of tiles: segments need only be disjoint, and each descriptor
+//    holds its block's record base).  Each block's 16-byte descriptor
+//    {record base, records, entry, map} is filed under its size class with
+//    one counter atomic per class and workgroup, and its pass counter is
+//    reset.  The descriptor holds the entry, not its heap block: the heap
+//    blocks of new entries are being assigned beside it, and the fold reads
+//    them.
 __global__ void __launch_bounds__(256) k_seg(Dev D) {
   __shared__ uint32_t sh[kScanSh];
   __shared__ int lcnt[FC_COUNT], gbase[FC_COUNT];
   __shared__ uint32_t carry_sh;
   const int n = ld_acq(&D.ctr->touched_n);
   const int nn = ld_acq(&D.ctr->new_n);  // (read with the other counts: one round trip)
-  const uint32_t epoch = (uint32_t)ld_acq(&D.ctr->pass_epoch);
   // heap failures include quota failures, which are also counted apart
-  // (columns of the SR_SCAN rows: ST_NEW_BLOCKS, ST_HEAP_FAIL, ST_QUOTA_FAIL,
-  // and records dropped by a tile whose wait reached its bound)
-  uint32_t st[4] = {0, 0, 0, 0};
+  // (columns of the SR_SCAN rows: ST_NEW_BLOCKS, ST_HEAP_FAIL, ST_QUOTA_FAIL)
+  uint32_t st[3] = {0, 0, 0};
   const int nchunks = (nn + 255) / 256;
   for (int c = gridDim.x - 1 - blockIdx.x; c < nchunks; c += gridDim.x) {
     const int x = c * 256 + threadIdx.x;
@@ -1140,33 +1123,9 @@ __global__ void __launch_bounds__(256) k_seg(Dev D) {
     if (threadIdx.x < FC_COUNT) lcnt[threadIdx.x] = 0;
     uint32_t tot;
     uint32_t ex = block_scan_excl(cn[0] + cn[1] + cn[2] + cn[3], sh, tot);
-    if (threadIdx.x == 0) {
-      uint32_t before = 0;
-      if (t == 0) {
-        __hip_atomic_store(&D.tile_state[0], tile_word(epoch, kTileUpTo, tot), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        __hip_atomic_store(&D.tile_state[t], tile_word(epoch, kTileOwn, tot), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        int j = t - 1;
-        for (int spins = 0;;) {
-          const unsigned long long w = __hip_atomic_load(&D.tile_state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          const uint32_t tag = (uint32_t)(w >> 32);
-          if ((tag >> 2) == (epoch & 0x3FFFFFFFu) && (tag & 3u) != 0) {
-            before += (uint32_t)w;
-            if ((tag & 3u) == kTileUpTo || j == 0) break;
-            --j;
-          } else if (++spins > kTileSpin) {
-            st[3] += tot;
-            before = 0;
-            break;
-          }
-        }
-        __hip_atomic_store(&D.tile_state[t], tile_word(epoch, kTileUpTo, before + tot), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      }
-      carry_sh = before;
-    }
+    // the tile's segments go to the next free records of the pass, in any
+    // order of tiles: each block's descriptor holds its own record base
+    if (threadIdx.x == 0) carry_sh = tot ? atomicAdd(&D.ctr->seg_top, tot) : 0u;
     __syncthreads();
     ex += carry_sh;
     int cls[4], rank[4];

@@ -184,7 +184,6 @@ struct tsdf_engine {
   int64_t seg_cap = 0;
   uint32_t* touched = nullptr;
   uint32_t* tile_sums = nullptr;
-  unsigned long long* tile_state = nullptr;
   uint32_t* fail_list = nullptr;
   uint32_t* new_list = nullptr;
   uint8_t* new_map = nullptr;
@@ -255,7 +254,6 @@ struct tsdf_engine {
     D.pcap = pcap;
     D.touched = touched;
     D.tile_sums = tile_sums;
-    D.tile_state = tile_state;
     D.fail_list = fail_list;
     D.new_list = new_list;
     D.new_map = new_map;
@@ -844,7 +842,7 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
       (r = dev_alloc(&E->heap_free, c.heap_blocks)) || (r = dev_alloc(&E->freed, c.heap_blocks)) ||
       (r = dev_alloc(&E->desc_dev, 1)) ||
       (r = dev_alloc(&E->touched, entries)) ||
-      (r = dev_alloc(&E->tile_sums, entries / kScanTile + 2)) || (r = dev_alloc(&E->tile_state, entries / kScanTile + 2)) ||
+      (r = dev_alloc(&E->tile_sums, entries / kScanTile + 2)) ||
       (r = dev_alloc(&E->fail_list, entries)) ||
       (r = dev_alloc(&E->new_list, entries)) || (r = dev_alloc(&E->new_map, entries)) ||
       (r = dev_alloc(&E->touched_map, entries)) || (r = dev_alloc(&E->part, ST_COUNT * kPartSlots)) || (r = dev_alloc(&E->srows, (int64_t)kStatRowsTotal * kStatCols)) ||
@@ -875,7 +873,6 @@ int tsdf_engine_create(const tsdf_engine_config* cfg, tsdf_engine** out) {
       hipMemsetAsync(E->mctr, 0, sizeof(MapCtr) * kMaxMaps, st) != hipSuccess ||
       hipMemsetAsync(E->heap, 0, (size_t)vbytes, st) != hipSuccess ||
       hipMemsetAsync(E->blk_cnt, 0, sizeof(unsigned long long) * entries, st) != hipSuccess ||
-      hipMemsetAsync(E->tile_state, 0, sizeof(unsigned long long) * (entries / kScanTile + 2), st) != hipSuccess ||
       hipMemsetAsync(E->part, 0, sizeof(unsigned long long) * ST_COUNT * kPartSlots, st) != hipSuccess ||
       hipMemsetAsync(E->srows, 0, sizeof(uint32_t) * kStatRowsTotal * kStatCols, st) != hipSuccess ||
       hipMemsetAsync(E->keys, 0xFF, sizeof(unsigned long long) * entries, st) != hipSuccess ||
@@ -906,7 +903,7 @@ int tsdf_engine_destroy(tsdf_engine* E) {
     }
   void* ptrs[] = {E->maps_dev, E->mctr, E->ctr, E->keys, E->vals, E->blk_cnt, E->blk_base, E->next, E->ovf_stack,
                   E->heap, E->heap_free, E->freed, E->desc_dev, E->pair_e, E->pair_r, E->pair_n,
-                  E->seg, E->touched, E->tile_sums, E->tile_state, E->fail_list, E->new_list, E->new_map, E->fold_desc, E->part, E->srows,
+                  E->seg, E->touched, E->tile_sums, E->fail_list, E->new_list, E->new_map, E->fold_desc, E->part, E->srows,
                   E->touched_map, E->sel,
                   E->pub_cnt, E->pub_off, E->pub_buf, E->in_buf};
   for (void* p : ptrs)
